@@ -1,0 +1,116 @@
+/*
+ * modarith.h — exact 64-bit modular arithmetic for the three NTT primes.
+ *
+ * Host and device.  The primes are p = c*2^k + 1 < 2^62 with k >= 41, so
+ * lazily reduced values in [0, 4p) still fit one u64 word (Harvey's lazy
+ * butterflies) and every power-of-two transform length up to 2^41 has roots.
+ * Arithmetic follows the reference's primitives (ntt/lib/primes.h:193-279,
+ * ntt/lib/shoup.h:45-112) with the build's primes and lazy ranges.
+ */
+#ifndef CRYPTOGRAPHY_AMD_MODARITH_H
+#define CRYPTOGRAPHY_AMD_MODARITH_H
+
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+
+#define NTT_NPRIMES 3
+/* Twiddles are powers of one root of order 2^NTT_ROOT_LOG per prime. */
+#define NTT_ROOT_LOG 30
+#define NTT_TW_LOG 15                      /* two-level table: 2^15 entries each */
+#define NTT_TW_SIZE (1u << NTT_TW_LOG)
+#define NTT_MAX_LOG_LEN NTT_ROOT_LOG       /* longest transform: 2^30 coefficients */
+
+namespace nttc {
+
+/* prime i: value, exponent k (p = c*2^k+1), generator of Z_p^* */
+__host__ __device__ constexpr uint64_t prime(int i)
+{
+    return i == 0 ? 0x3fffc00000000001ull : i == 1 ? 0x3fffbe0000000001ull : 0x3fff840000000001ull;
+}
+__host__ __device__ constexpr uint64_t prime_gen(int i) { return i == 0 ? 11 : i == 1 ? 3 : 19; }
+__host__ __device__ constexpr int prime_k(int i) { return i == 0 ? 46 : i == 1 ? 41 : 42; }
+
+/* -p^-1 mod 2^64 by Newton iteration (each step doubles the correct bits). */
+__host__ __device__ constexpr uint64_t neg_inv64(uint64_t p)
+{
+    uint64_t x = p;                 /* p*p == 1 mod 8 for odd p: 3 bits */
+    x *= 2 - p * x; x *= 2 - p * x; x *= 2 - p * x; x *= 2 - p * x; x *= 2 - p * x;
+    return (uint64_t)0 - x;
+}
+
+__host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
+#endif
+}
+
+template <int PI>
+struct Prime {
+    static constexpr uint64_t P = prime(PI);
+    static constexpr uint64_t P2 = 2 * prime(PI);
+    static constexpr uint64_t NINV = neg_inv64(prime(PI));   /* -p^-1 mod 2^64 */
+};
+
+/* x in [0, 2p) -> [0, p) */
+template <int PI>
+__host__ __device__ __forceinline__ uint64_t reduce1(uint64_t x)
+{
+    return x >= Prime<PI>::P ? x - Prime<PI>::P : x;
+}
+/* x in [0, 4p) -> [0, 2p) */
+template <int PI>
+__host__ __device__ __forceinline__ uint64_t reduce2(uint64_t x)
+{
+    return x >= Prime<PI>::P2 ? x - Prime<PI>::P2 : x;
+}
+
+/* Shoup product a*w mod p for any a < 2^64, w < p, wp = floor(w*2^64/p);
+ * result in [0, 2p).  q = hi(a*wp) is floor(a*w/p) or one less. */
+template <int PI>
+__host__ __device__ __forceinline__ uint64_t shoup_lazy(uint64_t a, uint64_t w, uint64_t wp)
+{
+    uint64_t q = mulhi64(a, wp);
+    return a * w - q * Prime<PI>::P;
+}
+
+/* Montgomery product a*b*2^-64 mod p for a*b < 2^64 * p; result in [0, p). */
+template <int PI>
+__host__ __device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b)
+{
+    uint64_t tlo = a * b, thi = mulhi64(a, b);
+    uint64_t m = tlo * Prime<PI>::NINV;
+    uint64_t mphi = mulhi64(m, Prime<PI>::P);
+    /* t + m*p is divisible by 2^64; its low word sum carries iff tlo != 0 */
+    uint64_t u = thi + mphi + (tlo != 0);
+    return u >= Prime<PI>::P ? u - Prime<PI>::P : u;
+}
+
+/* Forward (decimation-in-frequency) butterfly on [0,2p) values:
+ * (x, y) -> (x + y, (x - y) * w), both outputs in [0, 2p). */
+template <int PI>
+__host__ __device__ __forceinline__ void dif_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wp)
+{
+    uint64_t s = x + y;
+    uint64_t d = x - y + Prime<PI>::P2;
+    x = reduce2<PI>(s);
+    y = shoup_lazy<PI>(d, w, wp);
+}
+
+/* Inverse (decimation-in-time) butterfly on [0,2p) values:
+ * (x, y) -> (x + y*w, x - y*w), both outputs in [0, 2p). */
+template <int PI>
+__host__ __device__ __forceinline__ void dit_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wp)
+{
+    uint64_t t = shoup_lazy<PI>(y, w, wp);
+    uint64_t s = x + t;
+    uint64_t d = x - t + Prime<PI>::P2;
+    x = reduce2<PI>(s);
+    y = reduce2<PI>(d);
+}
+
+}  // namespace nttc
+
+#endif

@@ -1,0 +1,504 @@
+/*
+ * ntt_kernels.hip — gfx950 kernels of the 3-prime CRT-NTT big-integer multiply.
+ *
+ * Data layout (HBM): for one product of transform length L = 2^logL the
+ * residues live in one u64 array res[op][prime][L] (op 0 = a, op 1 = b).
+ * Transforms are in place.  The forward transform is a multi-pass
+ * decimation-in-frequency (DIF) four-step: pass i views the current block of
+ * N = S*T elements as an S x T matrix (element (m, r) at r + m*T), runs the
+ * S-point DIF sub-transform down every column in LDS, multiplies element
+ * (m, r) by the cross twiddle w_N^(r*bitrev(m)), and leaves T-point blocks for
+ * the next pass.  Spectra end up in a scrambled order that the inverse (the
+ * mirror-image decimation-in-time passes) consumes directly, so no transpose
+ * or bit-reversal pass exists anywhere.  Values between kernels are lazily
+ * reduced to [0, 2p).
+ *
+ * Reference counterparts: ntt/lib/ntt_kernel.hip:625-675 (stockham_kernel,
+ * LDS sub-NTT), :880-918 (xtranspose_fwd/inv, cross twiddle), :386-395
+ * (hadamard_kernel), ntt/lib/garner.hip:187-260 (garner_kernel),
+ * ntt/lib/transfer_kernels.hip:55-313 (gather / carry-lookahead kernels) and
+ * ntt/lib/arith/ntt_mul.hip:117-143 (scatter_kernel).
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "modarith.h"
+#include "ntt_kernels.h"
+
+using namespace nttc;
+
+#define TPB 256  /* threads per block for the transform passes */
+
+/* ------------------------------------------------------------------------ */
+/* Twiddle tables: tw[prime][0][j] = w^j, tw[prime][1][j] = w^(j*2^15) for a  */
+/* root w of order 2^30, each with its Shoup partner floor(t*2^64/p).         */
+/* ------------------------------------------------------------------------ */
+
+template <int PI>
+__device__ uint64_t mulmod_dev(uint64_t a, uint64_t b, uint64_t r2)
+{
+    /* a*b*2^-64 then *2^128*2^-64: exact a*b mod p for a, b < p */
+    return mont_mul<PI>(mont_mul<PI>(a, b), r2);
+}
+
+template <int PI>
+__device__ uint64_t powmod_dev(uint64_t b, uint64_t e, uint64_t r2)
+{
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = mulmod_dev<PI>(r, b, r2);
+        b = mulmod_dev<PI>(b, b, r2);
+        e >>= 1;
+    }
+    return r;
+}
+
+/* floor(w * 2^64 / p) for w < p < 2^62 by restoring long division. */
+template <int PI>
+__device__ uint64_t shoup_partner(uint64_t w)
+{
+    const uint64_t p = Prime<PI>::P;
+    uint64_t r = w, q = 0;
+    for (int i = 63; i >= 0; i--) {
+        r <<= 1;                       /* r < p < 2^62: no overflow */
+        if (r >= p) { r -= p; q |= (uint64_t)1 << i; }
+    }
+    return q;
+}
+
+template <int PI>
+__device__ void twiddle_gen_one(TwPair *tw, uint32_t j, uint64_t r2)
+{
+    const uint64_t p = Prime<PI>::P;
+    uint64_t w = powmod_dev<PI>(prime_gen(PI), (p - 1) >> NTT_ROOT_LOG, r2);
+    uint64_t lo = powmod_dev<PI>(w, j, r2);
+    uint64_t hi = powmod_dev<PI>(w, (uint64_t)j << NTT_TW_LOG, r2);
+    TwPair *t = tw + (size_t)PI * 2 * NTT_TW_SIZE;
+    t[j].w = lo;
+    t[j].wp = shoup_partner<PI>(lo);
+    t[NTT_TW_SIZE + j].w = hi;
+    t[NTT_TW_SIZE + j].wp = shoup_partner<PI>(hi);
+}
+
+__global__ void __launch_bounds__(256) k_twiddle_gen(TwPair *tw, uint64_t r2_0, uint64_t r2_1, uint64_t r2_2)
+{
+    uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= NTT_TW_SIZE) return;
+    switch (blockIdx.y) {
+    case 0: twiddle_gen_one<0>(tw, j, r2_0); break;
+    case 1: twiddle_gen_one<1>(tw, j, r2_1); break;
+    default: twiddle_gen_one<2>(tw, j, r2_2); break;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Transform passes                                                           */
+/* ------------------------------------------------------------------------ */
+
+__device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits)
+{
+    return bits ? (__brev(x) >> (32 - bits)) : 0;
+}
+
+/* w_N^e for e < N = 2^logN: exponent E = e * 2^(30-logN) of the 2^30 root. */
+template <int PI>
+__device__ __forceinline__ uint64_t cross_twiddle(uint64_t x, const TwPair *tw, uint32_t E)
+{
+    TwPair lo = tw[E & (NTT_TW_SIZE - 1)];
+    TwPair hi = tw[NTT_TW_SIZE + (E >> NTT_TW_LOG)];
+    x = shoup_lazy<PI>(x, lo.w, lo.wp);
+    return shoup_lazy<PI>(x, hi.w, hi.wp);
+}
+
+struct TileGeom {
+    uint32_t logS, logT, logC, logG;
+    __device__ __forceinline__ uint32_t tile_log() const { return logS + logC + logG; }
+};
+
+/* Global element index of local tile element q of tile t. */
+__device__ __forceinline__ size_t tile_addr(const TileGeom &g, uint32_t t, uint32_t q,
+                                            uint32_t &m, uint32_t &r)
+{
+    uint32_t c = q & ((1u << g.logC) - 1);
+    m = (q >> g.logC) & ((1u << g.logS) - 1);
+    uint32_t gg = q >> (g.logC + g.logS);
+    uint32_t colTiles = g.logT - g.logC;                 /* log2 column tiles per block */
+    size_t b0 = ((size_t)(t >> colTiles)) << g.logG;
+    uint32_t col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+    r = col0 + c;
+    return ((b0 + gg) << (g.logS + g.logT)) + r + ((size_t)m << g.logT);
+}
+
+/* One forward DIF pass over tile t of residue array x (prime PI).
+ * src16 != nullptr: first pass, read 16-bit coefficients of the operand
+ * (n16 of them, zero beyond) instead of x. */
+template <int PI>
+__device__ void fwd_pass_tile(uint64_t *x, const uint16_t *src16, size_t n16, const TwPair *twall,
+                              TileGeom g, uint32_t t, uint64_t *lds)
+{
+    const TwPair *tw = twall + (size_t)PI * 2 * NTT_TW_SIZE;
+    const uint32_t tsz = 1u << g.tile_log();
+    for (uint32_t q = threadIdx.x; q < tsz; q += TPB) {
+        uint32_t m, r;
+        size_t a = tile_addr(g, t, q, m, r);
+        lds[q] = src16 ? (a < n16 ? (uint64_t)src16[a] : 0) : x[a];
+    }
+    __syncthreads();
+    const uint32_t S = 1u << g.logS, C = 1u << g.logC;
+    for (int lh = (int)g.logS - 1; lh >= 0; lh--) {
+        uint32_t h = 1u << lh;
+        for (uint32_t b = threadIdx.x; b < tsz / 2; b += TPB) {
+            uint32_t c = b & (C - 1);
+            uint32_t u = b >> g.logC;
+            uint32_t gg = u >> (g.logS - 1);
+            uint32_t v = u & (S / 2 - 1);
+            uint32_t j = v & (h - 1);
+            uint32_t m1 = ((v >> lh) << (lh + 1)) + j;
+            uint32_t i1 = (((gg << g.logS) + m1) << g.logC) + c;
+            uint32_t i2 = i1 + (h << g.logC);
+            TwPair w = tw[NTT_TW_SIZE + (j << (NTT_TW_LOG - 1 - lh))];
+            uint64_t a0 = lds[i1], a1 = lds[i2];
+            dif_bfly<PI>(a0, a1, w.w, w.wp);
+            lds[i1] = a0;
+            lds[i2] = a1;
+        }
+        __syncthreads();
+    }
+    const uint32_t logN = g.logS + g.logT;
+    for (uint32_t q = threadIdx.x; q < tsz; q += TPB) {
+        uint32_t m, r;
+        size_t a = tile_addr(g, t, q, m, r);
+        uint64_t v = lds[q];
+        if (g.logT) {
+            uint32_t E = (r * bitrev(m, g.logS)) << (NTT_ROOT_LOG - logN);
+            v = cross_twiddle<PI>(v, tw, E);
+        }
+        x[a] = v;
+    }
+}
+
+/* One inverse DIT pass (mirror of fwd_pass_tile). */
+template <int PI>
+__device__ void inv_pass_tile(uint64_t *x, const TwPair *twall, TileGeom g, uint32_t t, uint64_t *lds)
+{
+    const TwPair *tw = twall + (size_t)PI * 2 * NTT_TW_SIZE;
+    const uint32_t tsz = 1u << g.tile_log();
+    const uint32_t logN = g.logS + g.logT;
+    for (uint32_t q = threadIdx.x; q < tsz; q += TPB) {
+        uint32_t m, r;
+        size_t a = tile_addr(g, t, q, m, r);
+        uint64_t v = x[a];
+        if (g.logT) {
+            uint32_t E = (r * bitrev(m, g.logS)) << (NTT_ROOT_LOG - logN);
+            E = (0u - E) & ((1u << NTT_ROOT_LOG) - 1);
+            v = cross_twiddle<PI>(v, tw, E);
+        }
+        lds[q] = v;
+    }
+    __syncthreads();
+    const uint32_t S = 1u << g.logS, C = 1u << g.logC;
+    for (uint32_t lh = 0; lh < g.logS; lh++) {
+        uint32_t h = 1u << lh;
+        for (uint32_t b = threadIdx.x; b < tsz / 2; b += TPB) {
+            uint32_t c = b & (C - 1);
+            uint32_t u = b >> g.logC;
+            uint32_t gg = u >> (g.logS - 1);
+            uint32_t v = u & (S / 2 - 1);
+            uint32_t j = v & (h - 1);
+            uint32_t m1 = ((v >> lh) << (lh + 1)) + j;
+            uint32_t i1 = (((gg << g.logS) + m1) << g.logC) + c;
+            uint32_t i2 = i1 + (h << g.logC);
+            uint32_t e = (NTT_TW_SIZE - (j << (NTT_TW_LOG - 1 - lh))) & (NTT_TW_SIZE - 1);
+            TwPair w = tw[NTT_TW_SIZE + e];
+            uint64_t a0 = lds[i1], a1 = lds[i2];
+            dit_bfly<PI>(a0, a1, w.w, w.wp);
+            lds[i1] = a0;
+            lds[i2] = a1;
+        }
+        __syncthreads();
+    }
+    for (uint32_t q = threadIdx.x; q < tsz; q += TPB) {
+        uint32_t m, r;
+        size_t a = tile_addr(g, t, q, m, r);
+        x[a] = lds[q];
+    }
+}
+
+/* grid: x = tile, y = prime, z = operand.  LDS: 2^tile_log u64. */
+__global__ void __launch_bounds__(TPB) k_fwd_pass(PassArgs pa)
+{
+    extern __shared__ uint64_t lds[];
+    const uint32_t pi = blockIdx.y, op = blockIdx.z;
+    uint64_t *x = pa.res + ((size_t)op * NTT_NPRIMES + pi) * pa.len;
+    const uint16_t *src = pa.first ? pa.src16[op] : nullptr;
+    size_t n16 = pa.first ? pa.n16[op] : 0;
+    TileGeom g{pa.logS, pa.logT, pa.logC, pa.logG};
+    switch (pi) {
+    case 0: fwd_pass_tile<0>(x, src, n16, pa.tw, g, blockIdx.x, lds); break;
+    case 1: fwd_pass_tile<1>(x, src, n16, pa.tw, g, blockIdx.x, lds); break;
+    default: fwd_pass_tile<2>(x, src, n16, pa.tw, g, blockIdx.x, lds); break;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_inv_pass(PassArgs pa)
+{
+    extern __shared__ uint64_t lds[];
+    const uint32_t pi = blockIdx.y;
+    uint64_t *x = pa.res + (size_t)pi * pa.len;
+    TileGeom g{pa.logS, pa.logT, pa.logC, pa.logG};
+    switch (pi) {
+    case 0: inv_pass_tile<0>(x, pa.tw, g, blockIdx.x, lds); break;
+    case 1: inv_pass_tile<1>(x, pa.tw, g, blockIdx.x, lds); break;
+    default: inv_pass_tile<2>(x, pa.tw, g, blockIdx.x, lds); break;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pointwise product with the 1/L scale: a[i] = a[i] * b[i] / L mod p.        */
+/* ------------------------------------------------------------------------ */
+
+template <int PI>
+__device__ __forceinline__ uint64_t pw_one(uint64_t a, uint64_t b, uint64_t s, uint64_t sp)
+{
+    /* mont_mul gives a*b/2^64; s = 2^64/L mod p restores the scale */
+    return shoup_lazy<PI>(mont_mul<PI>(a, b), s, sp);
+}
+
+__global__ void __launch_bounds__(256) k_pointwise(PointwiseArgs pw)
+{
+    const uint32_t pi = blockIdx.y;
+    uint64_t *a = pw.res + (size_t)pi * pw.len;
+    const uint64_t *b = pw.square ? a : pw.res + ((size_t)NTT_NPRIMES + pi) * pw.len;
+    const uint64_t s = pw.scale[pi], sp = pw.scale_p[pi];
+    size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < pw.len; i += stride) {
+        uint64_t va = a[i], vb = b[i];
+        uint64_t r;
+        switch (pi) {
+        case 0: r = pw_one<0>(va, vb, s, sp); break;
+        case 1: r = pw_one<1>(va, vb, s, sp); break;
+        default: r = pw_one<2>(va, vb, s, sp); break;
+        }
+        a[i] = r;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Garner CRT + 16-bit repack + carry lookahead                               */
+/* ------------------------------------------------------------------------ */
+
+/* x = a0 + a1*p0 + a2*p0*p1 in the reference's order (garner.hip:135-173);
+ * the product coefficients are < L*(2^16-1)^2 < 2^62 < p0, so a1 = a2 = 0 for
+ * an intact product and the value is a0.  Any nonzero a1/a2 is a corrupted
+ * product and raises the integrity flag. */
+__device__ __forceinline__ uint64_t garner_coeff(const uint64_t *r0, const uint64_t *r1, const uint64_t *r2,
+                                                 size_t k, const GarnerConsts &gc, int &bad)
+{
+    const uint64_t p0 = Prime<0>::P, p1 = Prime<1>::P, p2 = Prime<2>::P;
+    uint64_t x0 = reduce1<0>(r0[k]);
+    uint64_t x1 = reduce1<1>(r1[k]);
+    uint64_t x2 = reduce1<2>(r2[k]);
+    uint64_t a0 = x0;
+    uint64_t a0m1 = a0 >= p1 ? a0 - p1 : a0;            /* a0 < p0 < 2*p1 */
+    uint64_t t1 = x1 >= a0m1 ? x1 - a0m1 : x1 + p1 - a0m1;
+    uint64_t a1 = reduce1<1>(shoup_lazy<1>(t1, gc.inv01, gc.inv01p));
+    uint64_t a0m2 = a0 >= p2 ? a0 - p2 : a0;
+    uint64_t b2 = a0m2 + reduce1<2>(shoup_lazy<2>(a1, gc.p0m2, gc.p0m2p));
+    b2 = b2 >= p2 ? b2 - p2 : b2;
+    uint64_t t2 = x2 >= b2 ? x2 - b2 : x2 + p2 - b2;
+    uint64_t a2 = reduce1<2>(shoup_lazy<2>(t2, gc.inv012, gc.inv012p));
+    (void)p0;
+    bad |= (a1 | a2) != 0;
+    return a0;
+}
+
+/* K_a: each thread owns CARRY_E consecutive output limbs.  Writes the
+ * pre-carry digit res_j to out[j], the per-limb generate bits to gen[thread]
+ * and the block's (G,P) aggregate to blk[block]. */
+__global__ void __launch_bounds__(256) k_garner_pack(GarnerArgs ga)
+{
+    __shared__ uint8_t sg[256];
+    const size_t thr = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t j0 = thr * CARRY_E;
+    const uint64_t *r0 = ga.res, *r1 = ga.res + ga.len, *r2 = ga.res + 2 * ga.len;
+    int bad = 0;
+    uint8_t G = 0, P = 1, gbits = 0;
+    if (j0 < ga.n_out) {
+        uint64_t prev_hi = 0;
+        for (int jj = -1; jj < CARRY_E; jj++) {
+            if (jj == -1 && j0 == 0) continue;
+            size_t j = j0 + jj;
+            if (j >= ga.n_out) break;
+            uint64_t lo = 0, hi = 0;
+            for (int t = 0; t < 4; t++) {
+                size_t k = 4 * j + t;
+                if (k >= ga.n_coef) break;
+                uint64_t X = garner_coeff(r0, r1, r2, k, ga.gc, bad);
+                if (t == 0) { lo = X; }
+                else {
+                    uint64_t sl = X << (16 * t), sh = X >> (64 - 16 * t);
+                    uint64_t nl = lo + sl;
+                    hi += sh + (nl < lo);
+                    lo = nl;
+                }
+            }
+            if (jj >= 0) {
+                uint64_t res = lo + prev_hi;
+                uint8_t g = res < lo;
+                uint8_t p = res == ~0ull;
+                ga.out[j] = res;
+                gbits |= (uint8_t)(g << jj);
+                G = g | (p & G);
+                P = P & p;
+            }
+            prev_hi = hi;
+        }
+        /* the carry out of the top limb must vanish */
+        if (j0 + CARRY_E >= ga.n_out && j0 < ga.n_out) bad |= prev_hi != 0;
+        ga.gen[thr] = gbits;
+    }
+    if (bad) ga.flag[0] = 1;
+    sg[threadIdx.x] = (uint8_t)(G | (P << 1));
+    __syncthreads();
+    /* block aggregate: fold thread cells in order (lower index first) */
+    for (int s = 1; s < 256; s <<= 1) {
+        uint8_t cur = sg[threadIdx.x];
+        uint8_t lo = (threadIdx.x >= (unsigned)s) ? sg[threadIdx.x - s] : (uint8_t)2;  /* identity (0,1) */
+        __syncthreads();
+        uint8_t g = (cur & 1) | ((cur >> 1) & lo & 1);
+        uint8_t p = (uint8_t)((cur >> 1) & (lo >> 1) & 1);
+        sg[threadIdx.x] = (uint8_t)(g | (p << 1));
+        __syncthreads();
+    }
+    if (threadIdx.x == 255) ga.blk[blockIdx.x] = sg[255];
+}
+
+/* K_b: exclusive carry-in for every block (one workgroup, 1024 threads). */
+__global__ void __launch_bounds__(1024) k_carry_scan(uint8_t *blk, size_t nblk)
+{
+    __shared__ uint8_t sg[1024];
+    const size_t per = (nblk + 1023) / 1024;
+    const size_t s0 = threadIdx.x * per;
+    uint8_t G = 0, P = 1;
+    for (size_t i = s0; i < s0 + per && i < nblk; i++) {
+        uint8_t c = blk[i];
+        uint8_t g = c & 1, p = (c >> 1) & 1;
+        G = g | (p & G);
+        P = P & p;
+    }
+    sg[threadIdx.x] = (uint8_t)(G | (P << 1));
+    __syncthreads();
+    for (int s = 1; s < 1024; s <<= 1) {
+        uint8_t cur = sg[threadIdx.x];
+        uint8_t lo = (threadIdx.x >= (unsigned)s) ? sg[threadIdx.x - s] : (uint8_t)2;
+        __syncthreads();
+        uint8_t g = (cur & 1) | ((cur >> 1) & lo & 1);
+        uint8_t p = (uint8_t)((cur >> 1) & (lo >> 1) & 1);
+        sg[threadIdx.x] = (uint8_t)(g | (p << 1));
+        __syncthreads();
+    }
+    /* carry into this thread's segment = G of the inclusive prefix of the previous thread */
+    uint8_t cin = threadIdx.x ? (sg[threadIdx.x - 1] & 1) : 0;
+    __syncthreads();
+    for (size_t i = s0; i < s0 + per && i < nblk; i++) {
+        uint8_t c = blk[i];
+        blk[i] = cin;                 /* becomes the block's carry-in */
+        cin = (c & 1) | (((c >> 1) & 1) & cin);
+    }
+}
+
+/* K_c: apply carries.  Thread carry-in = block carry-in composed with the
+ * in-block exclusive prefix; then ripple through the thread's limbs. */
+__global__ void __launch_bounds__(256) k_carry_apply(GarnerArgs ga)
+{
+    __shared__ uint8_t sg[256];
+    const size_t thr = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t j0 = thr * CARRY_E;
+    uint8_t G = 0, P = 1, gbits = 0;
+    uint64_t v[CARRY_E];
+    if (j0 < ga.n_out) {
+        gbits = ga.gen[thr];
+        for (int jj = 0; jj < CARRY_E; jj++) {
+            size_t j = j0 + jj;
+            v[jj] = j < ga.n_out ? ga.out[j] : 0;
+            uint8_t g = (gbits >> jj) & 1;
+            uint8_t p = j < ga.n_out && v[jj] == ~0ull;
+            G = g | (p & G);
+            P = P & p;
+        }
+    }
+    sg[threadIdx.x] = (uint8_t)(G | (P << 1));
+    __syncthreads();
+    for (int s = 1; s < 256; s <<= 1) {
+        uint8_t cur = sg[threadIdx.x];
+        uint8_t lo = (threadIdx.x >= (unsigned)s) ? sg[threadIdx.x - s] : (uint8_t)2;
+        __syncthreads();
+        uint8_t g = (cur & 1) | ((cur >> 1) & lo & 1);
+        uint8_t p = (uint8_t)((cur >> 1) & (lo >> 1) & 1);
+        sg[threadIdx.x] = (uint8_t)(g | (p << 1));
+        __syncthreads();
+    }
+    uint8_t bc = ga.blk[blockIdx.x];
+    uint8_t cin;
+    if (threadIdx.x == 0) cin = bc;
+    else {
+        uint8_t e = sg[threadIdx.x - 1];
+        cin = (e & 1) | ((e >> 1) & bc);
+    }
+    if (j0 < ga.n_out) {
+        for (int jj = 0; jj < CARRY_E; jj++) {
+            size_t j = j0 + jj;
+            if (j >= ga.n_out) break;
+            uint64_t r = v[jj];
+            uint8_t g = (gbits >> jj) & 1;
+            uint8_t p = r == ~0ull;
+            ga.out[j] = r + cin;
+            cin = g | (p & cin);
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Schoolbook for small operands: one workgroup, column sums then a ripple.   */
+/* ------------------------------------------------------------------------ */
+
+__global__ void __launch_bounds__(256) k_schoolbook(const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
+                                                    uint64_t *out)
+{
+    extern __shared__ uint64_t col[];   /* 3 words per output column */
+    const size_t nout = na + nb;
+    for (size_t j = threadIdx.x; j < nout; j += 256) {
+        uint64_t w0 = 0, w1 = 0, w2 = 0;
+        size_t ilo = j >= nb ? j - nb + 1 : 0;
+        size_t ihi = j < na ? j : na - 1;
+        for (size_t i = ilo; i <= ihi && j < nout - 1; i++) {
+            uint64_t x = a[i], y = b[j - i];
+            uint64_t lo = x * y, hi = __umul64hi(x, y);
+            uint64_t s = w0 + lo;
+            uint64_t c = s < w0;
+            w0 = s;
+            uint64_t s1 = w1 + hi;
+            uint64_t c1 = s1 < w1;
+            uint64_t s2 = s1 + c;
+            c1 += s2 < s1;
+            w1 = s2;
+            w2 += c1;
+        }
+        col[3 * j] = w0; col[3 * j + 1] = w1; col[3 * j + 2] = w2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t c0 = 0, c1 = 0;   /* running carry (up to 2 words) */
+        for (size_t j = 0; j < nout; j++) {
+            uint64_t s = col[3 * j] + c0;
+            uint64_t k = s < c0;
+            uint64_t n1 = col[3 * j + 1] + c1;
+            uint64_t k1 = n1 < c1;
+            uint64_t n1b = n1 + k;
+            k1 += n1b < n1;
+            out[j] = s;
+            c0 = n1b;
+            c1 = col[3 * j + 2] + k1;
+        }
+    }
+}

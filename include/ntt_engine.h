@@ -1,0 +1,95 @@
+#pragma once
+/*
+ * ntt_engine.h — C ABI of the MI355X (gfx950) 3-prime CRT-NTT multiply engine.
+ *
+ * Status-returning entry points.  The reference's fail-stop entry points
+ * (multiply.h: bigint_mul, ntt_bigint_mul, ntt_mul_init, ...) are thin
+ * wrappers over these that print the error and exit(1) as the reference does.
+ *
+ * Operands and products are little-endian u64 limb arrays (a u64 limb array
+ * is also the 32-bit-limb array the coefficient split reads).  A product of
+ * na and nb limbs always has na + nb limbs (untrimmed).
+ *
+ * Threading: one engine per process on one GPU.  Every entry point takes the
+ * engine lock, so concurrent callers serialize, as the reference's
+ * ntt_gpu_lock does (ntt/lib/arith/multiply.c:22-27,86-88).
+ */
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NTT_OK = 0,
+    NTT_ERR_NOT_INIT = 1,     /* engine could not be initialised (no GPU, HIP error at init) */
+    NTT_ERR_TOO_LARGE = 2,    /* transform would exceed 2^30 coefficients (cf. ntt_mul.hip:1013-1022) */
+    NTT_ERR_HIP = 3,          /* a HIP runtime call failed (cf. HIP_CHECK, ntt_mul.hip:101-109) */
+    NTT_ERR_ARG = 4,          /* invalid argument (null pointer with nonzero size, ...) */
+    NTT_ERR_INTEGRITY = 5,    /* CRT integrity check failed: a corrupted product */
+    NTT_ERR_NOMEM = 6         /* device or host allocation failed */
+} ntt_status;
+
+/* Message for the last error on this engine (never NULL). */
+const char *ntt_engine_last_error(void);
+
+/* Initialise on HIP device `device` (idempotent).  Builds the twiddle and
+ * Shoup tables on the device.  No counterpart needed by callers of the
+ * reference API: ntt_mul_init calls it. */
+int ntt_engine_init(int device);
+/* Release every device buffer and the stream (cf. ntt_mul_teardown_impl). */
+void ntt_engine_teardown(void);
+int ntt_engine_is_init(void);
+
+/* log2 of the transform length used for a product of na and nb limbs
+ * (16-bit coefficients: least power of two >= 4(na+nb)). */
+int ntt_engine_log_len(size_t na, size_t nb);
+
+/* Host-buffer multiply / square: c[0..na+nb) = a*b.  Synchronous. */
+int ntt_mul_limbs(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb);
+int ntt_sqr_limbs(uint64_t *c, const uint64_t *a, size_t na);
+
+/* Device-resident multiply / square on the engine stream.  d_c, d_a, d_b are
+ * device pointers; d_c must not alias d_a or d_b.  Synchronous: returns after
+ * the product is complete and its CRT integrity check has been read. */
+int ntt_mul_device(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb);
+int ntt_sqr_device(uint64_t *d_c, const uint64_t *d_a, size_t na);
+/* Same, enqueued only (no host wait, no integrity read); pair with
+ * ntt_engine_sync(), which waits and reports a failed integrity check. */
+int ntt_mul_device_async(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb);
+int ntt_sqr_device_async(uint64_t *d_c, const uint64_t *d_a, size_t na);
+int ntt_engine_sync(void);
+
+/* Schoolbook / NTT crossover in limbs (both operands at or below it use the
+ * GPU schoolbook kernel).  Cf. multiply.h:23-31. */
+int ntt_engine_get_threshold(void);
+void ntt_engine_set_threshold(int limbs);
+
+/* Number of device multiplies issued since init (cf. ntt_mul.hip:595). */
+unsigned long ntt_engine_dispatch_count(void);
+
+/* Timing of the NTT stages (forward passes, pointwise, inverse passes) of the
+ * last device multiply, measured with HIP events on the engine stream;
+ * enabled by ntt_engine_set_timing(1).  Milliseconds; 0 when disabled. */
+void ntt_engine_set_timing(int on);
+double ntt_engine_last_ntt_ms(void);
+double ntt_engine_last_total_ms(void);
+/* Per-launch durations of the last NTT multiply (timing enabled): up to max
+ * entries; names[32*i] holds kernel i's name, ms[i] its duration.  Returns
+ * the count. */
+int ntt_engine_kernel_times(char *names, double *ms, int max);
+
+/* Harness helpers: device memory owned by the caller. */
+void *ntt_device_alloc(size_t bytes);
+void ntt_device_free(void *p);
+int ntt_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int ntt_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* Copy of the on-device twiddle table (3 primes x {w^j, w^(j 2^15)} x 2^15
+ * Shoup pairs of u64, i.e. 3*2*2^15*2 words) for verification. */
+int ntt_engine_read_twiddles(uint64_t *out, size_t n_words);
+
+#ifdef __cplusplus
+}
+#endif

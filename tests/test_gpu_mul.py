@@ -1,0 +1,132 @@
+"""GPU parity: the HIP CRT-NTT multiply against the oracle, GMP and Python ints.
+
+Every call goes through the C ABI of libntt_amd.so.  Results must be bit-exact.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import cryptography_amd as E
+
+pytestmark = pytest.mark.gpu
+
+
+def pyint_mul(a, b):
+    return O.int_to_limbs(O.limbs_to_int(a) * O.limbs_to_int(b), len(a) + len(b))
+
+
+def test_twiddle_tables_on_device():
+    E.init()
+    n = 3 * 2 * (1 << 15) * 2
+    buf = np.zeros(n, dtype=np.uint64)
+    E.engine.check(E.lib().ntt_engine_read_twiddles(buf.ctypes.data_as(E.engine._u64p), n))
+    t = buf.reshape(3, 2, 1 << 15, 2)
+    rng = random.Random(3)
+    for pi, p in enumerate(O.primes()):
+        w = O.orc().orc_root(pi, 30)
+        for j in [0, 1, 2, 12345, (1 << 15) - 1] + [rng.randrange(1 << 15) for _ in range(50)]:
+            lo, hi = pow(w, j, p), pow(w, j << 15, p)
+            assert int(t[pi, 0, j, 0]) == lo and int(t[pi, 0, j, 1]) == (lo << 64) // p
+            assert int(t[pi, 1, j, 0]) == hi and int(t[pi, 1, j, 1]) == (hi << 64) // p
+
+
+SIZES = [(1, 1), (1, 2), (2, 1), (3, 3), (64, 64), (65, 64), (64, 65), (100, 7), (7, 100),
+         (128, 128), (255, 1), (1000, 1000), (1024, 1024), (1025, 1023), (4096, 17), (5000, 4999),
+         (16384, 16384), (40000, 3)]
+
+
+@pytest.mark.parametrize("na,nb", SIZES)
+def test_mul_random_vs_oracle(na, nb):
+    a = O.splitmix(na * 1000 + nb, na)
+    b = O.splitmix(nb * 7919 + na + 1, nb)
+    got = E.mul(a, b)
+    want = O.gmp_mul(a, b)
+    assert np.array_equal(got, want)
+    if na + nb <= 4096:
+        assert np.array_equal(got, O.mul_ntt3(a, b))
+
+
+@pytest.mark.parametrize("na,nb", [(65, 65), (300, 2000), (4096, 4096)])
+def test_ntt_path_forced_small(na, nb):
+    """Threshold 0 forces the NTT path even for small operands."""
+    E.init()
+    L = E.lib()
+    saved = L.ntt_engine_get_threshold()
+    L.ntt_engine_set_threshold(0)
+    try:
+        for na_, nb_ in [(1, 1), (2, 3), (na, nb)]:
+            a = O.splitmix(na_ + 5, na_)
+            b = O.splitmix(nb_ + 6, nb_)
+            assert np.array_equal(E.mul(a, b), pyint_mul(a, b))
+    finally:
+        L.ntt_engine_set_threshold(saved)
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 1024, 4096, 65536])
+def test_all_ones_operands(n):
+    a = np.full(n, 0xFFFFFFFFFFFFFFFF, np.uint64)
+    want = pyint_mul(a, a)
+    assert np.array_equal(E.mul(a, a), want)
+    assert np.array_equal(E.sqr(a), want)
+
+
+@pytest.mark.parametrize("n", [1, 5, 64, 1000, 8192, 70000])
+def test_square_vs_mul(n):
+    a = O.splitmix(n + 99, n)
+    got = E.sqr(a)
+    assert np.array_equal(got, O.gmp_mul(a, a))
+
+
+def test_zero_and_sparse_operands():
+    z = np.zeros(1000, np.uint64)
+    a = O.splitmix(1, 1000)
+    assert np.array_equal(E.mul(z, a), np.zeros(2000, np.uint64))
+    # powers of two and single-bit operands: long carry chains after the convolution
+    x = np.zeros(3000, np.uint64); x[-1] = 1 << 63
+    y = np.full(2500, 0xFFFFFFFFFFFFFFFF, np.uint64)
+    assert np.array_equal(E.mul(x, y), pyint_mul(x, y))
+    assert np.array_equal(E.mul(np.zeros(0, np.uint64), a), np.zeros(1000, np.uint64))
+
+
+def test_carry_chain_across_blocks():
+    # (2^(64n) - 1) * (2^(64m) + 1): ripples a carry over > 2048 limbs (several carry blocks)
+    n = 9000
+    a = np.full(n, 0xFFFFFFFFFFFFFFFF, np.uint64)
+    b = np.zeros(n, np.uint64); b[0] = 1; b[-1] = 1
+    assert np.array_equal(E.mul(a, b), pyint_mul(a, b))
+
+
+def test_config1_2_22_bit_product():
+    """2^22-bit operands, L = 2^19: bit-exact vs GMP and the CPU oracle."""
+    a = O.random_operand(1, 1 << 22)
+    b = O.random_operand(2, 1 << 22)
+    assert E.log_len(len(a), len(b)) == 19
+    got = E.mul(a, b)
+    assert np.array_equal(got, O.gmp_mul(a, b))
+    assert np.array_equal(got, O.mul_ntt3(a, b, threads=3))
+
+
+def test_device_api_and_async():
+    a = O.random_operand(5, 1 << 20)
+    b = O.random_operand(6, 1 << 19)
+    want = O.gmp_mul(a, b)
+    with E.DeviceBuffer.from_array(a) as da, E.DeviceBuffer.from_array(b) as db, \
+            E.DeviceBuffer(len(a) + len(b)) as dc:
+        E.mul_device(dc, da, len(a), db, len(b))
+        assert np.array_equal(dc.download(), want)
+        for _ in range(3):
+            E.mul_device(dc, da, len(a), db, len(b), sync=False)
+        E.engine.check(E.lib().ntt_engine_sync())
+        assert np.array_equal(dc.download(), want)
+        E.sqr_device(dc, da, len(a) // 2)
+        h = a[: len(a) // 2]
+        assert np.array_equal(dc.download(len(a)), O.gmp_mul(h, h))
+
+
+def test_too_large_is_an_error():
+    E.init()
+    with pytest.raises(E.NttError) as ei:
+        E.mul_device(E.DeviceBuffer(1), E.DeviceBuffer(1), 1 << 28, E.DeviceBuffer(1), 1 << 28)
+    assert ei.value.code == 2
