@@ -1,0 +1,101 @@
+/*
+ * engine_internal.h — engine state and executor functions shared by the
+ * engine's translation units (ntt_engine.hip, tree.hip).  Not part of the ABI.
+ */
+#ifndef CRYPTOGRAPHY_AMD_ENGINE_INTERNAL_H
+#define CRYPTOGRAPHY_AMD_ENGINE_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/ntt_engine.h"
+#include "ntt_kernels.h"
+
+#define NTT_MAX_MARKS 40
+
+namespace ntte {
+
+struct Engine {
+    int init = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    TwPair *tw = nullptr;
+    int *flag = nullptr;
+    MulDesc *d_desc = nullptr;      /* descriptor array of the last batch call */
+    size_t d_desc_n = 0;
+    /* host staging for the host-buffer API */
+    uint64_t *d_a = nullptr, *d_b = nullptr, *d_c = nullptr;
+    size_t d_a_words = 0, d_b_words = 0, d_c_words = 0;
+    int threshold = 64;
+    unsigned long dispatches = 0;
+    int timing = 0;
+    /* timing marks: one event after each launch of the last multiply */
+    hipEvent_t ev[NTT_MAX_MARKS] = {};
+    const char *mark_name[NTT_MAX_MARKS] = {};
+    int n_marks = 0, ntt_mark = 0;
+    double mark_ms[NTT_MAX_MARKS] = {};
+    double last_ntt_ms = 0, last_total_ms = 0;
+    int timing_pending = 0;
+    GarnerConsts gc;
+    char err[512] = "";
+};
+
+extern Engine g;
+extern std::recursive_mutex g_lock;
+
+/* Device workspace of one executor (the engine's own, or a graph's). */
+struct Workspace {
+    uint64_t *res = nullptr;
+    size_t res_words = 0;
+    uint8_t *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    void release()
+    {
+        if (res) (void)hipFree(res);
+        if (scratch) (void)hipFree(scratch);
+        res = nullptr;
+        scratch = nullptr;
+        res_words = scratch_bytes = 0;
+    }
+};
+
+extern Workspace g_ws;
+
+struct PassPlan {
+    int npass;
+    uint32_t logS[8];
+};
+
+struct BatchPlan {
+    struct Class {
+        int logL;            /* 0: schoolbook */
+        uint32_t first, count, max_nout;
+    };
+    std::vector<MulDesc> sorted;     /* descriptors grouped by class */
+    std::vector<Class> classes;
+    size_t res_words = 0, scratch = 0;
+};
+
+int fail(int code, const char *fmt, ...);
+#define HIP_TRY(x)                                                                       \
+    do {                                                                                 \
+        hipError_t e_ = (x);                                                             \
+        if (e_ != hipSuccess)                                                            \
+            return fail(NTT_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                             \
+    } while (0)
+
+hipError_t mark(const char *name);
+int grow_dev(uint64_t **buf, size_t *have, size_t words);
+int ws_reserve(Workspace &ws, size_t res_words, size_t scratch_bytes);
+int log_len(size_t na, size_t nb);
+int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);
+int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hipStream_t st);
+int sync_check();
+
+}  // namespace ntte
+
+#endif

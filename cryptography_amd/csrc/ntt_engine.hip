@@ -13,48 +13,23 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
+#include <vector>
 
 #include "../../include/ntt_engine.h"
 #include "modarith.h"
 #include "ntt_kernels.h"
+#include "engine_internal.h"
 
 using namespace nttc;
 
-#define NTT_MAX_MARKS 40
-
-namespace {
-
-struct Engine {
-    int init = 0;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    TwPair *tw = nullptr;
-    /* workspace (grow-only) */
-    uint64_t *res = nullptr;
-    size_t res_words = 0;
-    uint8_t *scratch = nullptr;     /* gen bytes + block bytes */
-    size_t scratch_bytes = 0;
-    int *flag = nullptr;
-    /* host staging for the host-buffer API */
-    uint64_t *d_a = nullptr, *d_b = nullptr, *d_c = nullptr;
-    size_t d_a_words = 0, d_b_words = 0, d_c_words = 0;
-    int threshold = 64;
-    unsigned long dispatches = 0;
-    int timing = 0;
-    /* timing marks: one event after each launch of the last multiply */
-    hipEvent_t ev[NTT_MAX_MARKS] = {};
-    const char *mark_name[NTT_MAX_MARKS] = {};
-    int n_marks = 0, ntt_mark = 0;
-    double mark_ms[NTT_MAX_MARKS] = {};
-    double last_ntt_ms = 0, last_total_ms = 0;
-    int timing_pending = 0;
-    GarnerConsts gc;
-    char err[512] = "";
-};
+namespace ntte {
 
 Engine g;
 std::recursive_mutex g_lock;
+
+Workspace g_ws;
 
 /* Record a timing event named for the launch that precedes it. */
 hipError_t mark(const char *name)
@@ -72,14 +47,6 @@ int fail(int code, const char *fmt, ...)
     va_end(ap);
     return code;
 }
-
-#define HIP_TRY(x)                                                                       \
-    do {                                                                                 \
-        hipError_t e_ = (x);                                                             \
-        if (e_ != hipSuccess)                                                            \
-            return fail(NTT_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                             \
-    } while (0)
 
 typedef unsigned __int128 u128;
 
@@ -110,18 +77,6 @@ int grow_dev(uint64_t **buf, size_t *have, size_t words)
         return fail(NTT_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", words * 8, hipGetErrorString(e));
     }
     *have = words;
-    return NTT_OK;
-}
-
-int ensure_scratch(size_t bytes)
-{
-    if (g.scratch_bytes >= bytes) return NTT_OK;
-    if (g.scratch) (void)hipFree(g.scratch);
-    g.scratch = nullptr;
-    g.scratch_bytes = 0;
-    hipError_t e = hipMalloc((void **)&g.scratch, bytes);
-    if (e != hipSuccess) return fail(NTT_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
-    g.scratch_bytes = bytes;
     return NTT_OK;
 }
 
@@ -166,8 +121,8 @@ void engine_teardown()
     (void)hipStreamSynchronize(g.stream);
     (void)hipFree(g.tw);
     (void)hipFree(g.flag);
-    if (g.res) (void)hipFree(g.res);
-    if (g.scratch) (void)hipFree(g.scratch);
+    g_ws.release();
+    if (g.d_desc) (void)hipFree(g.d_desc);
     if (g.d_a) (void)hipFree(g.d_a);
     if (g.d_b) (void)hipFree(g.d_b);
     if (g.d_c) (void)hipFree(g.d_c);
@@ -186,11 +141,6 @@ int log_len(size_t na, size_t nb)
     while (((size_t)1 << l) < need) l++;
     return l;
 }
-
-struct PassPlan {
-    int npass;
-    uint32_t logS[8];
-};
 
 PassPlan plan_passes(int logL)
 {
@@ -212,52 +162,64 @@ __global__ void k_fill_zero(uint64_t *p, size_t n)
     if (i < n) p[i] = 0;
 }
 
-/* Enqueue c = a*b (or a^2) on the engine stream.  Device pointers. */
-int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int square)
-{
-    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
-    if ((na && !d_a) || (nb && !d_b) || ((na + nb) && !d_c)) return fail(NTT_ERR_ARG, "null operand pointer");
-    const size_t nout = na + nb;
-    g.dispatches++;
-    if (na == 0 || nb == 0) {
-        if (nout)
-            hipLaunchKernelGGL(k_fill_zero, dim3((nout + 255) / 256), dim3(256), 0, g.stream, d_c, nout);
-        HIP_TRY(hipGetLastError());
-        return NTT_OK;
-    }
-    if ((int64_t)na <= g.threshold && (int64_t)nb <= g.threshold && nout <= 2048) {
-        hipLaunchKernelGGL(k_schoolbook, dim3(1), dim3(256), 3 * nout * 8, g.stream, d_a, na, d_b, nb, d_c);
-        HIP_TRY(hipGetLastError());
-        return NTT_OK;
-    }
-    const int logL = log_len(na, nb);
-    if (logL > NTT_MAX_LOG_LEN)
-        return fail(NTT_ERR_TOO_LARGE, "product of %zu x %zu limbs needs 2^%d coefficients (max 2^%d)", na, nb,
-                    logL, NTT_MAX_LOG_LEN);
-    const size_t L = (size_t)1 << logL;
-    const int nops = square ? 1 : 2;
-    int rc = grow_dev(&g.res, &g.res_words, (size_t)nops * NTT_NPRIMES * L);
-    if (rc) return rc;
-    const size_t nthr = (nout + CARRY_E - 1) / CARRY_E;
-    const size_t nblk = (nthr + 255) / 256;
-    rc = ensure_scratch(nthr + nblk + 64);
-    if (rc) return rc;
 
+int ws_reserve(Workspace &ws, size_t res_words, size_t scratch_bytes)
+{
+    int rc = grow_dev(&ws.res, &ws.res_words, res_words);
+    if (rc) return rc;
+    if (ws.scratch_bytes < scratch_bytes) {
+        if (ws.scratch) (void)hipFree(ws.scratch);
+        ws.scratch = nullptr;
+        ws.scratch_bytes = 0;
+        hipError_t e = hipMalloc((void **)&ws.scratch, scratch_bytes);
+        if (e != hipSuccess)
+            return fail(NTT_ERR_NOMEM, "hipMalloc(%zu) failed: %s", scratch_bytes, hipGetErrorString(e));
+        ws.scratch_bytes = scratch_bytes;
+    }
+    return NTT_OK;
+}
+
+/* Carry geometry of a class whose largest product has max_nout limbs. */
+void carry_geom(uint32_t max_nout, uint32_t &thr, uint32_t &blk)
+{
+    thr = (max_nout + CARRY_E - 1) / CARRY_E;
+    blk = (thr + 255) / 256;
+    thr = blk * 256;
+}
+
+/* Workspace a class needs: residues and carry scratch. */
+void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t &res_words, size_t &scratch)
+{
+    uint32_t thr, blk;
+    carry_geom(max_nout, thr, blk);
+    res_words = (size_t)(square ? 1 : 2) * NTT_NPRIMES * nprod * ((size_t)1 << logL);
+    scratch = (size_t)nprod * (thr + blk) + 64;
+}
+
+/* Enqueue one size class: nprod products, transform length 2^logL, all on
+ * `st`.  d_desc == nullptr means the single product `one`. */
+int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
+                      uint32_t max_nout, int square, hipStream_t st)
+{
+    const size_t L = (size_t)1 << logL;
+    const size_t len = L * nprod;
+    const int nops = square ? 1 : 2;
+    size_t need_res, need_scr;
+    class_needs(logL, nprod, max_nout, square, need_res, need_scr);
+    if (ws.res_words < need_res || ws.scratch_bytes < need_scr)
+        return fail(NTT_ERR_ARG, "workspace too small for the class (internal)");
     const PassPlan pp = plan_passes(logL);
     const uint32_t tile_log = logL < 12 ? logL : 12;
-    const size_t ntiles = L >> tile_log;
+    const size_t ntiles = len >> tile_log;
     const size_t lds = (size_t)8 << tile_log;
 
-    g.n_marks = 0;
-    HIP_TRY(mark("start"));
     PassArgs pa{};
-    pa.res = g.res;
+    pa.res = ws.res;
     pa.tw = g.tw;
-    pa.src16[0] = (const uint16_t *)d_a;
-    pa.src16[1] = (const uint16_t *)(square ? d_a : d_b);
-    pa.n16[0] = 4 * na;
-    pa.n16[1] = 4 * (square ? na : nb);
-    pa.len = L;
+    pa.desc = d_desc;
+    pa.one = one;
+    pa.len = len;
+    pa.logL = logL;
     uint32_t done = 0;
     for (int i = 0; i < pp.npass; i++) {
         pa.logS = pp.logS[i];
@@ -265,13 +227,13 @@ int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d
         pa.logC = pa.logT < tile_log - pa.logS ? pa.logT : tile_log - pa.logS;
         pa.logG = tile_log - pa.logS - pa.logC;
         pa.first = i == 0;
-        hipLaunchKernelGGL(k_fwd_pass, dim3(ntiles, NTT_NPRIMES, nops), dim3(256), lds, g.stream, pa);
+        hipLaunchKernelGGL(k_fwd_pass, dim3(ntiles, NTT_NPRIMES, nops), dim3(256), lds, st, pa);
         HIP_TRY(mark("k_fwd_pass"));
         done += pa.logS;
     }
     PointwiseArgs pw{};
-    pw.res = g.res;
-    pw.len = L;
+    pw.res = ws.res;
+    pw.len = len;
     pw.square = square;
     for (int i = 0; i < NTT_NPRIMES; i++) {
         uint64_t p = prime(i);
@@ -279,8 +241,8 @@ int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d
         pw.scale[i] = h_mulmod(r, h_inv(L % p, p), p);
         pw.scale_p[i] = h_shoup(pw.scale[i], p);
     }
-    size_t pwb = L / 256 < 8192 ? (L + 255) / 256 : 8192;
-    hipLaunchKernelGGL(k_pointwise, dim3(pwb, NTT_NPRIMES), dim3(256), 0, g.stream, pw);
+    size_t pwb = len / 256 < 8192 ? (len + 255) / 256 : 8192;
+    hipLaunchKernelGGL(k_pointwise, dim3(pwb, NTT_NPRIMES), dim3(256), 0, st, pw);
     HIP_TRY(mark("k_pointwise"));
     for (int i = pp.npass - 1; i >= 0; i--) {
         done -= pp.logS[i];
@@ -289,28 +251,160 @@ int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d
         pa.logC = pa.logT < tile_log - pa.logS ? pa.logT : tile_log - pa.logS;
         pa.logG = tile_log - pa.logS - pa.logC;
         pa.first = 0;
-        hipLaunchKernelGGL(k_inv_pass, dim3(ntiles, NTT_NPRIMES, 1), dim3(256), lds, g.stream, pa);
+        hipLaunchKernelGGL(k_inv_pass, dim3(ntiles, NTT_NPRIMES, 1), dim3(256), lds, st, pa);
         HIP_TRY(mark("k_inv_pass"));
     }
     g.ntt_mark = g.n_marks - 1;
     GarnerArgs ga{};
-    ga.res = g.res;
-    ga.len = L;
-    ga.n_coef = 4 * nout < L ? 4 * nout : L;
-    ga.n_out = nout;
-    ga.out = d_c;
-    ga.gen = g.scratch;
-    ga.blk = g.scratch + nthr;
+    ga.res = ws.res;
+    ga.len = len;
+    ga.logL = logL;
+    ga.desc = d_desc;
+    ga.one = one;
+    carry_geom(max_nout, ga.thr_per_prod, ga.blk_per_prod);
+    ga.gen = ws.scratch;
+    ga.blk = ws.scratch + (size_t)nprod * ga.thr_per_prod;
     ga.flag = g.flag;
     ga.gc = g.gc;
-    hipLaunchKernelGGL(k_garner_pack, dim3(nblk), dim3(256), 0, g.stream, ga);
+    hipLaunchKernelGGL(k_garner_pack, dim3(ga.blk_per_prod, nprod), dim3(256), 0, st, ga);
     HIP_TRY(mark("k_garner_pack"));
-    hipLaunchKernelGGL(k_carry_scan, dim3(1), dim3(1024), 0, g.stream, ga.blk, nblk);
+    hipLaunchKernelGGL(k_carry_scan, dim3(nprod), dim3(1024), 0, st, ga.blk, ga.blk_per_prod);
     HIP_TRY(mark("k_carry_scan"));
-    hipLaunchKernelGGL(k_carry_apply, dim3(nblk), dim3(256), 0, g.stream, ga);
+    hipLaunchKernelGGL(k_carry_apply, dim3(ga.blk_per_prod, nprod), dim3(256), 0, st, ga);
     HIP_TRY(mark("k_carry_apply"));
-    g.timing_pending = g.timing;
     HIP_TRY(hipGetLastError());
+    return NTT_OK;
+}
+
+int enqueue_schoolbook(const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_schoolbook, dim3((nprod + 3) / 4), dim3(256), 0, st, d_desc, one, nprod, g.flag);
+    HIP_TRY(mark("k_schoolbook"));
+    HIP_TRY(hipGetLastError());
+    return NTT_OK;
+}
+
+bool use_schoolbook(size_t na, size_t nb, size_t n_out)
+{
+    (void)n_out;   /* the kernel's LDS holds na + nb column sums */
+    return (int64_t)na <= g.threshold && (int64_t)nb <= g.threshold && na + nb <= SB_MAX_OUT;
+}
+
+int check_desc(const MulDesc &d)
+{
+    if ((d.na && !d.a) || (d.nb && !d.b) || (d.n_out && !d.c) || (d.nadd && !d.add))
+        return fail(NTT_ERR_ARG, "null pointer in a product descriptor");
+    /* n_out may be below na + nb when the caller knows the value is shorter
+     * (operands with leading zero limbs); anything that does not fit is an
+     * integrity error, never a silent truncation */
+    if (d.n_out < d.nadd || (d.n_out == 0 && d.na && d.nb))
+        return fail(NTT_ERR_ARG, "n_out=%u too small for the %u-limb addend", d.n_out, d.nadd);
+    if (log_len(d.na, d.nb) > NTT_MAX_LOG_LEN)
+        return fail(NTT_ERR_TOO_LARGE, "product of %u x %u limbs needs 2^%d coefficients (max 2^%d)", d.na, d.nb,
+                    log_len(d.na, d.nb), NTT_MAX_LOG_LEN);
+    return NTT_OK;
+}
+
+/* Enqueue c = a*b (or a^2) on the engine stream.  Device pointers. */
+int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int square)
+{
+    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    if ((na && !d_a) || (nb && !d_b) || ((na + nb) && !d_c)) return fail(NTT_ERR_ARG, "null operand pointer");
+    if (na > 0xffffffffu || nb > 0xffffffffu) return fail(NTT_ERR_TOO_LARGE, "operand too large");
+    const size_t nout = na + nb;
+    if (log_len(na, nb) > NTT_MAX_LOG_LEN)
+        return fail(NTT_ERR_TOO_LARGE, "product of %zu x %zu limbs needs 2^%d coefficients (max 2^%d)", na, nb,
+                    log_len(na, nb), NTT_MAX_LOG_LEN);
+    g.dispatches++;
+    g.n_marks = 0;
+    HIP_TRY(mark("start"));
+    if (na == 0 || nb == 0) {
+        if (nout)
+            hipLaunchKernelGGL(k_fill_zero, dim3((nout + 255) / 256), dim3(256), 0, g.stream, d_c, nout);
+        HIP_TRY(hipGetLastError());
+        return NTT_OK;
+    }
+    MulDesc one{};
+    one.a = d_a;
+    one.b = d_b;
+    one.c = d_c;
+    one.na = (uint32_t)na;
+    one.nb = (uint32_t)nb;
+    one.n_out = (uint32_t)nout;
+    if (use_schoolbook(na, nb, nout)) return enqueue_schoolbook(nullptr, one, 1, g.stream);
+    const int logL = log_len(na, nb);
+    size_t rw, sb;
+    class_needs(logL, 1, (uint32_t)nout, square, rw, sb);
+    int rc = ws_reserve(g_ws, rw, sb);
+    if (rc) return rc;
+    rc = enqueue_ntt_class(g_ws, nullptr, one, 1, logL, (uint32_t)nout, square, g.stream);
+    g.timing_pending = g.timing;
+    return rc;
+}
+
+/* ---- batches ------------------------------------------------------------- */
+
+int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
+{
+    static_assert(sizeof(ntt_mul_desc) == sizeof(MulDesc), "descriptor layouts differ");
+    std::vector<std::pair<int, size_t>> key(n);
+    for (size_t i = 0; i < n; i++) {
+        MulDesc d;
+        memcpy(&d, &h[i], sizeof d);
+        int rc = check_desc(d);
+        if (rc) return rc;
+        int k = (d.na == 0 || d.nb == 0) ? -1 : use_schoolbook(d.na, d.nb, d.n_out) ? 0 : log_len(d.na, d.nb);
+        key[i] = {k, i};
+    }
+    std::stable_sort(key.begin(), key.end());
+    bp.sorted.resize(n);
+    bp.classes.clear();
+    for (size_t i = 0; i < n; i++) {
+        memcpy(&bp.sorted[i], &h[key[i].second], sizeof(MulDesc));
+        const MulDesc &d = bp.sorted[i];
+        if (bp.classes.empty() || bp.classes.back().logL != key[i].first)
+            bp.classes.push_back({key[i].first, (uint32_t)i, 0, 0});
+        auto &c = bp.classes.back();
+        c.count++;
+        c.max_nout = std::max(c.max_nout, std::max(d.n_out, d.na + d.nb));
+    }
+    bp.res_words = bp.scratch = 0;
+    for (auto &c : bp.classes) {
+        if (c.logL <= 0) continue;
+        size_t rw, sb;
+        class_needs(c.logL, c.count, c.max_nout, 0, rw, sb);
+        bp.res_words = std::max(bp.res_words, rw);
+        bp.scratch = std::max(bp.scratch, sb);
+    }
+    return NTT_OK;
+}
+
+__global__ void k_zero_products(const MulDesc *desc, uint32_t n)
+{
+    /* products with an empty operand: c = addend (or 0) */
+    const MulDesc d = desc[blockIdx.y];
+    for (size_t j = (size_t)blockIdx.x * 256 + threadIdx.x; j < d.n_out; j += (size_t)gridDim.x * 256)
+        d.c[j] = (d.add && j < d.nadd) ? d.add[j] : 0;
+    (void)n;
+}
+
+/* Enqueue every class of a planned batch whose descriptors are at d_desc. */
+int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hipStream_t st)
+{
+    for (auto &c : bp.classes) {
+        const MulDesc *dd = d_desc + c.first;
+        int rc;
+        if (c.logL < 0) {
+            hipLaunchKernelGGL(k_zero_products, dim3(16, c.count), dim3(256), 0, st, dd, c.count);
+            HIP_TRY(hipGetLastError());
+            rc = NTT_OK;
+        } else if (c.logL == 0) {
+            rc = enqueue_schoolbook(dd, MulDesc{}, c.count, st);
+        } else {
+            rc = enqueue_ntt_class(ws, dd, MulDesc{}, c.count, c.logL, c.max_nout, 0, st);
+        }
+        if (rc) return rc;
+    }
     return NTT_OK;
 }
 
@@ -358,7 +452,9 @@ int host_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_
     return sync_check();
 }
 
-}  // namespace
+}  // namespace ntte
+
+using namespace ntte;
 
 extern "C" {
 
@@ -423,6 +519,117 @@ int ntt_sqr_device(uint64_t *d_c, const uint64_t *d_a, size_t na)
     std::lock_guard<std::recursive_mutex> lk(g_lock);
     int rc = enqueue_mul(d_c, d_a, na, d_a, na, 1);
     return rc ? rc : sync_check();
+}
+
+int ntt_mul_batch_device_async(const ntt_mul_desc *descs, size_t n)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    if (n == 0) return NTT_OK;
+    if (!descs) return fail(NTT_ERR_ARG, "null descriptor array");
+    BatchPlan bp;
+    int rc = plan_batch(descs, n, bp);
+    if (rc) return rc;
+    /* the descriptor buffer may still be read by an earlier batch */
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    if (g.d_desc_n < n) {
+        if (g.d_desc) (void)hipFree(g.d_desc);
+        g.d_desc = nullptr;
+        g.d_desc_n = 0;
+        HIP_TRY(hipMalloc((void **)&g.d_desc, n * sizeof(MulDesc)));
+        g.d_desc_n = n;
+    }
+    HIP_TRY(hipMemcpy(g.d_desc, bp.sorted.data(), n * sizeof(MulDesc), hipMemcpyHostToDevice));
+    rc = ws_reserve(g_ws, bp.res_words, bp.scratch);
+    if (rc) return rc;
+    g.dispatches += n;
+    g.n_marks = 0;
+    HIP_TRY(mark("start"));
+    rc = enqueue_batch(g_ws, bp, g.d_desc, g.stream);
+    g.timing_pending = g.timing;
+    return rc;
+}
+
+int ntt_mul_batch_device(const ntt_mul_desc *descs, size_t n)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    int rc = ntt_mul_batch_device_async(descs, n);
+    return rc ? rc : sync_check();
+}
+
+struct ntt_batch_graph {
+    BatchPlan bp;
+    Workspace ws;
+    MulDesc *d_desc = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t n = 0;
+};
+
+void ntt_batch_graph_destroy(ntt_batch_graph *gr)
+{
+    if (!gr) return;
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (g.init) (void)hipStreamSynchronize(g.stream);
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    if (gr->d_desc) (void)hipFree(gr->d_desc);
+    gr->ws.release();
+    delete gr;
+}
+
+int ntt_batch_graph_create(const ntt_mul_desc *descs, size_t n, ntt_batch_graph **out)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (!out) return fail(NTT_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    if (n == 0 || !descs) return fail(NTT_ERR_ARG, "empty batch");
+    ntt_batch_graph *gr = new ntt_batch_graph();
+    gr->n = n;
+    int rc = plan_batch(descs, n, gr->bp);
+    if (!rc) rc = ws_reserve(gr->ws, gr->bp.res_words ? gr->bp.res_words : 1, gr->bp.scratch ? gr->bp.scratch : 64);
+    if (!rc && hipMalloc((void **)&gr->d_desc, n * sizeof(MulDesc)) != hipSuccess)
+        rc = fail(NTT_ERR_NOMEM, "descriptor allocation failed");
+    if (!rc && hipMemcpy(gr->d_desc, gr->bp.sorted.data(), n * sizeof(MulDesc), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(NTT_ERR_HIP, "descriptor upload failed");
+    if (!rc && hipStreamSynchronize(g.stream) != hipSuccess) rc = fail(NTT_ERR_HIP, "stream sync failed");
+    if (rc) {
+        ntt_batch_graph_destroy(gr);
+        return rc;
+    }
+    int saved_timing = g.timing;
+    g.timing = 0;   /* no event nodes in the graph */
+    hipError_t e = hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        g.timing = saved_timing;
+        ntt_batch_graph_destroy(gr);
+        return fail(NTT_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    rc = enqueue_batch(gr->ws, gr->bp, gr->d_desc, g.stream);
+    e = hipStreamEndCapture(g.stream, &gr->graph);
+    g.timing = saved_timing;
+    if (!rc && e != hipSuccess) rc = fail(NTT_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (!rc) {
+        e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = fail(NTT_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    if (rc) {
+        ntt_batch_graph_destroy(gr);
+        return rc;
+    }
+    *out = gr;
+    return NTT_OK;
+}
+
+int ntt_batch_graph_launch(ntt_batch_graph *gr)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    if (!gr || !gr->exec) return fail(NTT_ERR_ARG, "null graph");
+    g.dispatches += gr->n;
+    HIP_TRY(hipGraphLaunch(gr->exec, g.stream));
+    return NTT_OK;
 }
 
 int ntt_engine_get_threshold(void) { return g.threshold; }

@@ -16,12 +16,23 @@ struct __attribute__((aligned(16))) TwPair {
 
 #define CARRY_E 8   /* output limbs per thread in the carry kernels */
 
+/* One product of a batch: c[0..n_out) = a*b (+ add).  Device pointers. */
+struct MulDesc {
+    const uint64_t *a, *b;      /* operand limbs */
+    uint64_t *c;                /* product limbs */
+    const uint64_t *add;        /* optional addend limbs (nullptr: none) */
+    uint32_t na, nb, nadd, n_out;
+};
+
+/* A size class: nprod products sharing one transform length L = 2^logL.
+ * Residues of product i occupy [i*L, (i+1)*L) of each (op, prime) row. */
 struct PassArgs {
-    uint64_t *res;              /* res[op][prime][len] */
+    uint64_t *res;              /* res[op][prime][nprod * L] */
     const TwPair *tw;           /* twiddle tables, 3 primes x 2 x 2^15 */
-    const uint16_t *src16[2];   /* first forward pass: 16-bit coefficients per operand */
-    size_t n16[2];              /* coefficient counts (zero beyond) */
-    size_t len;                 /* transform length L */
+    const MulDesc *desc;        /* nprod descriptors (device), or nullptr: use one */
+    MulDesc one;
+    size_t len;                 /* nprod * L: row length */
+    uint32_t logL;
     uint32_t logS, logT, logC, logG;
     int first;
 };
@@ -39,12 +50,15 @@ struct GarnerConsts {
     uint64_t inv012, inv012p;    /* (p0 p1)^-1 mod p2 */
 };
 
+/* Garner + carry over a size class: grid.y = product. */
 struct GarnerArgs {
     const uint64_t *res;        /* res[prime][len], operand a's slot */
-    size_t len;
-    size_t n_coef;              /* coefficients that may be nonzero */
-    size_t n_out;               /* output limbs */
-    uint64_t *out;
+    size_t len;                 /* row length nprod * L */
+    uint32_t logL;
+    const MulDesc *desc;        /* or nullptr: use one */
+    MulDesc one;
+    uint32_t thr_per_prod;      /* carry threads reserved per product */
+    uint32_t blk_per_prod;      /* carry blocks per product */
     uint8_t *gen;               /* one byte of generate bits per thread */
     uint8_t *blk;               /* block aggregates, then block carry-ins */
     int *flag;                  /* integrity flag: set on a corrupted product */
@@ -56,8 +70,10 @@ __global__ void k_fwd_pass(PassArgs pa);
 __global__ void k_inv_pass(PassArgs pa);
 __global__ void k_pointwise(PointwiseArgs pw);
 __global__ void k_garner_pack(GarnerArgs ga);
-__global__ void k_carry_scan(uint8_t *blk, size_t nblk);
+__global__ void k_carry_scan(uint8_t *blk, uint32_t nblk);
 __global__ void k_carry_apply(GarnerArgs ga);
-__global__ void k_schoolbook(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out);
+/* Batched schoolbook: one wavefront per product, 4 products per block. */
+#define SB_MAX_OUT 256
+__global__ void k_schoolbook(const MulDesc *desc, MulDesc one, uint32_t nprod, int *flag);
 
 #endif

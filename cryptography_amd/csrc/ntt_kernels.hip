@@ -129,19 +129,32 @@ __device__ __forceinline__ size_t tile_addr(const TileGeom &g, uint32_t t, uint3
     return ((b0 + gg) << (g.logS + g.logT)) + r + ((size_t)m << g.logT);
 }
 
-/* One forward DIF pass over tile t of residue array x (prime PI).
- * src16 != nullptr: first pass, read 16-bit coefficients of the operand
- * (n16 of them, zero beyond) instead of x. */
-template <int PI>
-__device__ void fwd_pass_tile(uint64_t *x, const uint16_t *src16, size_t n16, const TwPair *twall,
-                              TileGeom g, uint32_t t, uint64_t *lds)
+/* 16-bit coefficient `loc` of operand op (0: a, 1: b) of a product. */
+__device__ __forceinline__ uint64_t coeff16(const MulDesc &d, int op, size_t loc)
 {
-    const TwPair *tw = twall + (size_t)PI * 2 * NTT_TW_SIZE;
+    const uint16_t *s = (const uint16_t *)(op ? d.b : d.a);
+    size_t n16 = 4 * (size_t)(op ? d.nb : d.na);
+    return loc < n16 ? (uint64_t)s[loc] : 0;
+}
+
+/* One forward DIF pass over tile t of residue array x (prime PI).  The first
+ * pass reads the operands' 16-bit coefficients instead of x. */
+template <int PI>
+__device__ void fwd_pass_tile(uint64_t *x, const PassArgs &pa, int op, TileGeom g, uint32_t t, uint64_t *lds)
+{
+    const TwPair *tw = pa.tw + (size_t)PI * 2 * NTT_TW_SIZE;
     const uint32_t tsz = 1u << g.tile_log();
+    const size_t lmask = ((size_t)1 << pa.logL) - 1;
     for (uint32_t q = threadIdx.x; q < tsz; q += TPB) {
         uint32_t m, r;
         size_t a = tile_addr(g, t, q, m, r);
-        lds[q] = src16 ? (a < n16 ? (uint64_t)src16[a] : 0) : x[a];
+        if (pa.first) {
+            /* first pass: read the operand's 16-bit pieces directly (the split) */
+            size_t prod = a >> pa.logL;
+            lds[q] = coeff16(pa.desc ? pa.desc[prod] : pa.one, op, a & lmask);
+        } else {
+            lds[q] = x[a];
+        }
     }
     __syncthreads();
     const uint32_t S = 1u << g.logS, C = 1u << g.logC;
@@ -230,13 +243,11 @@ __global__ void __launch_bounds__(TPB) k_fwd_pass(PassArgs pa)
     extern __shared__ uint64_t lds[];
     const uint32_t pi = blockIdx.y, op = blockIdx.z;
     uint64_t *x = pa.res + ((size_t)op * NTT_NPRIMES + pi) * pa.len;
-    const uint16_t *src = pa.first ? pa.src16[op] : nullptr;
-    size_t n16 = pa.first ? pa.n16[op] : 0;
     TileGeom g{pa.logS, pa.logT, pa.logC, pa.logG};
     switch (pi) {
-    case 0: fwd_pass_tile<0>(x, src, n16, pa.tw, g, blockIdx.x, lds); break;
-    case 1: fwd_pass_tile<1>(x, src, n16, pa.tw, g, blockIdx.x, lds); break;
-    default: fwd_pass_tile<2>(x, src, n16, pa.tw, g, blockIdx.x, lds); break;
+    case 0: fwd_pass_tile<0>(x, pa, op, g, blockIdx.x, lds); break;
+    case 1: fwd_pass_tile<1>(x, pa, op, g, blockIdx.x, lds); break;
+    default: fwd_pass_tile<2>(x, pa, op, g, blockIdx.x, lds); break;
     }
 }
 
@@ -312,193 +323,240 @@ __device__ __forceinline__ uint64_t garner_coeff(const uint64_t *r0, const uint6
     return a0;
 }
 
-/* K_a: each thread owns CARRY_E consecutive output limbs.  Writes the
- * pre-carry digit res_j to out[j], the per-limb generate bits to gen[thread]
- * and the block's (G,P) aggregate to blk[block]. */
+/* (G,P) cell composition: lower-index cell `lo` then higher `hi`;
+ * identity is (0,1), encoded G | P<<1 = 2. */
+__device__ __forceinline__ uint8_t cla_compose(uint8_t lo, uint8_t hi)
+{
+    uint8_t g = (hi & 1) | ((hi >> 1) & lo & 1);
+    uint8_t p = (uint8_t)((hi >> 1) & (lo >> 1) & 1);
+    return (uint8_t)(g | (p << 1));
+}
+
+/* Inclusive scan of 256 (or 1024) cells in LDS, lower thread index first. */
+template <int N>
+__device__ __forceinline__ void cla_block_scan(uint8_t *sg)
+{
+    for (int s = 1; s < N; s <<= 1) {
+        uint8_t cur = sg[threadIdx.x];
+        uint8_t lo = (threadIdx.x >= (unsigned)s) ? sg[threadIdx.x - s] : (uint8_t)2;
+        __syncthreads();
+        sg[threadIdx.x] = cla_compose(lo, cur);
+        __syncthreads();
+    }
+}
+
+/* K_a (grid: carry blocks x products): each thread owns CARRY_E consecutive
+ * output limbs of one product.  s_j = sum_t X_{4j+t} 2^(16t) (+ addend_j) is
+ * split into lo_j + hi_j 2^64; res_j = lo_j + hi_{j-1} mod 2^64 is written to
+ * out[j], its overflow bit (generate) to gen[], and the block's (G,P)
+ * aggregate to blk[]. */
 __global__ void __launch_bounds__(256) k_garner_pack(GarnerArgs ga)
 {
     __shared__ uint8_t sg[256];
+    const uint32_t prod = blockIdx.y;
+    const MulDesc d = ga.desc ? ga.desc[prod] : ga.one;
+    const size_t L = (size_t)1 << ga.logL;
+    const size_t off = (size_t)prod << ga.logL;
+    const uint64_t *r0 = ga.res + off, *r1 = ga.res + ga.len + off, *r2 = ga.res + 2 * ga.len + off;
+    const size_t n_out = d.n_out;
+    const size_t n_coef = 4 * ((size_t)d.na + d.nb) < L ? 4 * ((size_t)d.na + d.nb) : L;
     const size_t thr = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t j0 = thr * CARRY_E;
-    const uint64_t *r0 = ga.res, *r1 = ga.res + ga.len, *r2 = ga.res + 2 * ga.len;
     int bad = 0;
     uint8_t G = 0, P = 1, gbits = 0;
-    if (j0 < ga.n_out) {
+    /* limbs at or above n_out that the convolution reaches must be zero */
+    const size_t n_span = (n_coef + 3) / 4;
+    if (j0 >= n_out && j0 < n_span) {
+        for (size_t k = 4 * j0; k < 4 * (j0 + CARRY_E) && k < n_coef; k++)
+            bad |= garner_coeff(r0, r1, r2, k, ga.gc, bad) != 0;
+    }
+    if (j0 < n_out) {
         uint64_t prev_hi = 0;
         for (int jj = -1; jj < CARRY_E; jj++) {
             if (jj == -1 && j0 == 0) continue;
             size_t j = j0 + jj;
-            if (j >= ga.n_out) break;
+            if (j >= n_out) break;
             uint64_t lo = 0, hi = 0;
             for (int t = 0; t < 4; t++) {
                 size_t k = 4 * j + t;
-                if (k >= ga.n_coef) break;
+                if (k >= n_coef) break;
                 uint64_t X = garner_coeff(r0, r1, r2, k, ga.gc, bad);
-                if (t == 0) { lo = X; }
-                else {
+                if (t == 0) {
+                    lo = X;
+                } else {
                     uint64_t sl = X << (16 * t), sh = X >> (64 - 16 * t);
                     uint64_t nl = lo + sl;
                     hi += sh + (nl < lo);
                     lo = nl;
                 }
             }
+            if (d.add && j < d.nadd) {
+                uint64_t nl = lo + d.add[j];
+                hi += nl < lo;
+                lo = nl;
+            }
             if (jj >= 0) {
                 uint64_t res = lo + prev_hi;
                 uint8_t g = res < lo;
                 uint8_t p = res == ~0ull;
-                ga.out[j] = res;
+                d.c[j] = res;
                 gbits |= (uint8_t)(g << jj);
                 G = g | (p & G);
                 P = P & p;
             }
             prev_hi = hi;
         }
-        /* the carry out of the top limb must vanish */
-        if (j0 + CARRY_E >= ga.n_out && j0 < ga.n_out) bad |= prev_hi != 0;
-        ga.gen[thr] = gbits;
+        /* the high part of the top limb must vanish */
+        if (j0 + CARRY_E >= n_out) bad |= prev_hi != 0;
+        ga.gen[(size_t)prod * ga.thr_per_prod + thr] = gbits;
     }
     if (bad) ga.flag[0] = 1;
     sg[threadIdx.x] = (uint8_t)(G | (P << 1));
     __syncthreads();
-    /* block aggregate: fold thread cells in order (lower index first) */
-    for (int s = 1; s < 256; s <<= 1) {
-        uint8_t cur = sg[threadIdx.x];
-        uint8_t lo = (threadIdx.x >= (unsigned)s) ? sg[threadIdx.x - s] : (uint8_t)2;  /* identity (0,1) */
-        __syncthreads();
-        uint8_t g = (cur & 1) | ((cur >> 1) & lo & 1);
-        uint8_t p = (uint8_t)((cur >> 1) & (lo >> 1) & 1);
-        sg[threadIdx.x] = (uint8_t)(g | (p << 1));
-        __syncthreads();
-    }
-    if (threadIdx.x == 255) ga.blk[blockIdx.x] = sg[255];
+    cla_block_scan<256>(sg);
+    if (threadIdx.x == 255) ga.blk[(size_t)prod * ga.blk_per_prod + blockIdx.x] = sg[255];
 }
 
-/* K_b: exclusive carry-in for every block (one workgroup, 1024 threads). */
-__global__ void __launch_bounds__(1024) k_carry_scan(uint8_t *blk, size_t nblk)
+/* K_b (grid: products): exclusive carry-in of every block of one product. */
+__global__ void __launch_bounds__(1024) k_carry_scan(uint8_t *blkall, uint32_t nblk)
 {
     __shared__ uint8_t sg[1024];
-    const size_t per = (nblk + 1023) / 1024;
-    const size_t s0 = threadIdx.x * per;
-    uint8_t G = 0, P = 1;
-    for (size_t i = s0; i < s0 + per && i < nblk; i++) {
-        uint8_t c = blk[i];
-        uint8_t g = c & 1, p = (c >> 1) & 1;
-        G = g | (p & G);
-        P = P & p;
-    }
-    sg[threadIdx.x] = (uint8_t)(G | (P << 1));
+    uint8_t *blk = blkall + (size_t)blockIdx.x * nblk;
+    const uint32_t per = (nblk + 1023) / 1024;
+    const uint32_t s0 = threadIdx.x * per;
+    uint8_t agg = 2;
+    for (uint32_t i = s0; i < s0 + per && i < nblk; i++) agg = cla_compose(agg, blk[i]);
+    sg[threadIdx.x] = agg;
     __syncthreads();
-    for (int s = 1; s < 1024; s <<= 1) {
-        uint8_t cur = sg[threadIdx.x];
-        uint8_t lo = (threadIdx.x >= (unsigned)s) ? sg[threadIdx.x - s] : (uint8_t)2;
-        __syncthreads();
-        uint8_t g = (cur & 1) | ((cur >> 1) & lo & 1);
-        uint8_t p = (uint8_t)((cur >> 1) & (lo >> 1) & 1);
-        sg[threadIdx.x] = (uint8_t)(g | (p << 1));
-        __syncthreads();
-    }
-    /* carry into this thread's segment = G of the inclusive prefix of the previous thread */
+    cla_block_scan<1024>(sg);
     uint8_t cin = threadIdx.x ? (sg[threadIdx.x - 1] & 1) : 0;
     __syncthreads();
-    for (size_t i = s0; i < s0 + per && i < nblk; i++) {
+    for (uint32_t i = s0; i < s0 + per && i < nblk; i++) {
         uint8_t c = blk[i];
         blk[i] = cin;                 /* becomes the block's carry-in */
         cin = (c & 1) | (((c >> 1) & 1) & cin);
     }
 }
 
-/* K_c: apply carries.  Thread carry-in = block carry-in composed with the
- * in-block exclusive prefix; then ripple through the thread's limbs. */
+/* K_c (grid: carry blocks x products): thread carry-in = block carry-in
+ * composed with the in-block exclusive prefix; then ripple through the
+ * thread's limbs. */
 __global__ void __launch_bounds__(256) k_carry_apply(GarnerArgs ga)
 {
     __shared__ uint8_t sg[256];
+    const uint32_t prod = blockIdx.y;
+    const MulDesc d = ga.desc ? ga.desc[prod] : ga.one;
+    const size_t n_out = d.n_out;
     const size_t thr = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t j0 = thr * CARRY_E;
     uint8_t G = 0, P = 1, gbits = 0;
     uint64_t v[CARRY_E];
-    if (j0 < ga.n_out) {
-        gbits = ga.gen[thr];
+    if (j0 < n_out) {
+        gbits = ga.gen[(size_t)prod * ga.thr_per_prod + thr];
         for (int jj = 0; jj < CARRY_E; jj++) {
             size_t j = j0 + jj;
-            v[jj] = j < ga.n_out ? ga.out[j] : 0;
+            v[jj] = j < n_out ? d.c[j] : 0;
             uint8_t g = (gbits >> jj) & 1;
-            uint8_t p = j < ga.n_out && v[jj] == ~0ull;
+            uint8_t p = j < n_out && v[jj] == ~0ull;
             G = g | (p & G);
             P = P & p;
         }
     }
     sg[threadIdx.x] = (uint8_t)(G | (P << 1));
     __syncthreads();
-    for (int s = 1; s < 256; s <<= 1) {
-        uint8_t cur = sg[threadIdx.x];
-        uint8_t lo = (threadIdx.x >= (unsigned)s) ? sg[threadIdx.x - s] : (uint8_t)2;
-        __syncthreads();
-        uint8_t g = (cur & 1) | ((cur >> 1) & lo & 1);
-        uint8_t p = (uint8_t)((cur >> 1) & (lo >> 1) & 1);
-        sg[threadIdx.x] = (uint8_t)(g | (p << 1));
-        __syncthreads();
-    }
-    uint8_t bc = ga.blk[blockIdx.x];
+    cla_block_scan<256>(sg);
+    uint8_t bc = ga.blk[(size_t)prod * ga.blk_per_prod + blockIdx.x];
     uint8_t cin;
     if (threadIdx.x == 0) cin = bc;
     else {
         uint8_t e = sg[threadIdx.x - 1];
         cin = (e & 1) | ((e >> 1) & bc);
     }
-    if (j0 < ga.n_out) {
+    if (j0 < n_out) {
         for (int jj = 0; jj < CARRY_E; jj++) {
             size_t j = j0 + jj;
-            if (j >= ga.n_out) break;
+            if (j >= n_out) {
+                if (cin) ga.flag[0] = 1;     /* carry out of the top limb */
+                cin = 0;
+                break;
+            }
             uint64_t r = v[jj];
             uint8_t g = (gbits >> jj) & 1;
             uint8_t p = r == ~0ull;
-            ga.out[j] = r + cin;
+            d.c[j] = r + cin;
             cin = g | (p & cin);
         }
+        if (cin && j0 + CARRY_E == n_out) ga.flag[0] = 1;
     }
 }
 
 /* ------------------------------------------------------------------------ */
-/* Schoolbook for small operands: one workgroup, column sums then a ripple.   */
+/* Schoolbook for small operands (na + nb <= SB_MAX_OUT): one wavefront per    */
+/* product, 4 products per block.  Column sums in 3 words, then lane 0        */
+/* ripples the carry (ntt/lib/arith/multiply.c:35-67 semantics, on the GPU).  */
 /* ------------------------------------------------------------------------ */
 
-__global__ void __launch_bounds__(256) k_schoolbook(const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
-                                                    uint64_t *out)
+__global__ void __launch_bounds__(256) k_schoolbook(const MulDesc *desc, MulDesc one, uint32_t nprod, int *flag)
 {
-    extern __shared__ uint64_t col[];   /* 3 words per output column */
-    const size_t nout = na + nb;
-    for (size_t j = threadIdx.x; j < nout; j += 256) {
-        uint64_t w0 = 0, w1 = 0, w2 = 0;
-        size_t ilo = j >= nb ? j - nb + 1 : 0;
-        size_t ihi = j < na ? j : na - 1;
-        for (size_t i = ilo; i <= ihi && j < nout - 1; i++) {
-            uint64_t x = a[i], y = b[j - i];
-            uint64_t lo = x * y, hi = __umul64hi(x, y);
-            uint64_t s = w0 + lo;
-            uint64_t c = s < w0;
-            w0 = s;
-            uint64_t s1 = w1 + hi;
-            uint64_t c1 = s1 < w1;
-            uint64_t s2 = s1 + c;
-            c1 += s2 < s1;
-            w1 = s2;
-            w2 += c1;
+    __shared__ uint64_t colall[4][3 * SB_MAX_OUT];
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t prod = blockIdx.x * 4 + w;
+    const bool valid = prod < nprod;
+    MulDesc d = one;
+    if (valid && desc) d = desc[prod];
+    uint64_t *col = colall[w];
+    const size_t na = d.na, nb = d.nb, nout = na + nb;
+    if (valid) {
+        for (size_t j = lane; j < nout; j += 64) {
+            uint64_t w0 = 0, w1 = 0, w2 = 0;
+            size_t ilo = j >= nb ? j - nb + 1 : 0;
+            size_t ihi = j < na ? j : na - 1;
+            for (size_t i = ilo; i <= ihi && j + 1 < nout; i++) {
+                uint64_t x = d.a[i], y = d.b[j - i];
+                uint64_t lo = x * y, hi = __umul64hi(x, y);
+                uint64_t s0 = w0 + lo;
+                uint64_t c0 = s0 < w0;
+                w0 = s0;
+                uint64_t s1 = w1 + hi;
+                uint64_t c1 = s1 < w1;
+                uint64_t s2 = s1 + c0;
+                c1 += s2 < s1;
+                w1 = s2;
+                w2 += c1;
+            }
+            if (d.add && j < d.nadd) {
+                uint64_t s0 = w0 + d.add[j];
+                uint64_t c0 = s0 < w0;
+                w0 = s0;
+                uint64_t s1 = w1 + c0;
+                w2 += s1 < w1;
+                w1 = s1;
+            }
+            col[3 * j] = w0;
+            col[3 * j + 1] = w1;
+            col[3 * j + 2] = w2;
         }
-        col[3 * j] = w0; col[3 * j + 1] = w1; col[3 * j + 2] = w2;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint64_t c0 = 0, c1 = 0;   /* running carry (up to 2 words) */
-        for (size_t j = 0; j < nout; j++) {
-            uint64_t s = col[3 * j] + c0;
+    if (valid && lane == 0) {
+        uint64_t c0 = 0, c1 = 0;   /* running carry (two words) */
+        const size_t nspan = nout > d.n_out ? nout : d.n_out;
+        uint64_t spill = 0;
+        for (size_t j = 0; j < nspan; j++) {
+            uint64_t x0 = j < nout ? col[3 * j] : 0, x1 = j < nout ? col[3 * j + 1] : 0,
+                     x2 = j < nout ? col[3 * j + 2] : 0;
+            if (j >= nout && d.add && j < d.nadd) x0 = d.add[j];
+            uint64_t s = x0 + c0;
             uint64_t k = s < c0;
-            uint64_t n1 = col[3 * j + 1] + c1;
+            uint64_t n1 = x1 + c1;
             uint64_t k1 = n1 < c1;
             uint64_t n1b = n1 + k;
             k1 += n1b < n1;
-            out[j] = s;
+            if (j < d.n_out) d.c[j] = s;
+            else spill |= s;
             c0 = n1b;
-            c1 = col[3 * j + 2] + k1;
+            c1 = x2 + k1;
         }
+        if (c0 | c1 | spill) flag[0] = 1;   /* product does not fit n_out limbs */
     }
 }

@@ -1,0 +1,335 @@
+/*
+ * tree.hip — level-batched binary-splitting product tree for sum 1/k!.
+ *
+ * Same tree as the reference (ntt/app/compute_e/binary_split.c:74-133): the
+ * range [a, b) splits at m = a + (b-a)/2; a node computes B = B_L * B_R and
+ * P = P_L * B_R + P_R.  Instead of a depth-first recursion with one
+ * serialized multiply per call, every node of one height is evaluated in one
+ * batched launch sequence (both products of a node, the addend P_R fused into
+ * the carry pass), and the whole tree is captured into one hipGraph.
+ *
+ * Nodes whose range has at most LEAF_TERMS terms are evaluated directly by a
+ * leaf kernel (one thread per node): B = prod k, P by Horner
+ * (P = 1; P = P*j + 1 for j = a+1..b-1), which equals the reference's
+ * recursion on that range because P and B depend only on [a, b).
+ *
+ * Every value lives in one device arena at a fixed offset with a fixed
+ * capacity taken from an upper bound on its bits (lgamma), so the complete
+ * launch sequence is known before any work runs.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "engine_internal.h"
+
+using namespace ntte;
+
+#define LEAF_TERMS 16
+#define LEAF_LIMBS 8   /* 16 terms of < 32 bits: at most 512 bits */
+
+struct LeafDesc {
+    uint64_t a, b;           /* range [a, b) */
+    uint64_t offP, offB;     /* arena offsets (limbs) */
+    uint32_t capP, capB;     /* capacities (limbs) */
+};
+
+/* x[0..n) *= v; returns the carry limb */
+__device__ __forceinline__ uint64_t mul_small(uint64_t *x, int n, uint64_t v)
+{
+    uint64_t carry = 0;
+    for (int i = 0; i < n; i++) {
+        uint64_t lo = x[i] * v, hi = __umul64hi(x[i], v);
+        uint64_t s = lo + carry;
+        carry = hi + (s < lo);
+        x[i] = s;
+    }
+    return carry;
+}
+
+__global__ void __launch_bounds__(256) k_leaves(const LeafDesc *leaves, uint32_t n, uint64_t *arena)
+{
+    uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const LeafDesc d = leaves[i];
+    uint64_t P[LEAF_LIMBS], B[LEAF_LIMBS];
+    for (int k = 0; k < LEAF_LIMBS; k++) P[k] = B[k] = 0;
+    if (d.b <= d.a) {          /* empty range: P = 0, B = 1 (binary_split.c:84-90) */
+        B[0] = 1;
+    } else {
+        P[0] = 1;
+        B[0] = d.a;
+        for (uint64_t j = d.a + 1; j < d.b; j++) {
+            mul_small(B, LEAF_LIMBS, j);
+            mul_small(P, LEAF_LIMBS, j);
+            /* + 1 */
+            for (int k = 0; k < LEAF_LIMBS; k++) {
+                if (++P[k] != 0) break;
+            }
+        }
+    }
+    for (uint32_t k = 0; k < d.capP; k++) arena[d.offP + k] = k < LEAF_LIMBS ? P[k] : 0;
+    for (uint32_t k = 0; k < d.capB; k++) arena[d.offB + k] = k < LEAF_LIMBS ? B[k] : 0;
+}
+
+struct TreeNode {
+    long a, b;
+    int left = -1, right = -1;  /* child node indices; -1 for a leaf */
+    int height = 0;
+    uint64_t offP = 0, offB = 0;
+    uint32_t capP = 0, capB = 0;
+};
+
+/* Upper bound on the limbs of B = prod_{k=a}^{b-1} k (and of P < 2B). */
+static uint32_t cap_limbs_B(long a, long b)
+{
+    if (b <= a) return 1;
+    double bits = (lgamma((double)b) - lgamma((double)a)) / log(2.0);
+    return (uint32_t)(bits / 64.0 + 1e-9 * bits) + 2;
+}
+
+static int build_nodes(long a, long b, std::vector<TreeNode> &nodes)
+{
+    TreeNode nd;
+    nd.a = a;
+    nd.b = b;
+    int idx = (int)nodes.size();
+    nodes.push_back(nd);
+    if (b - a > LEAF_TERMS) {
+        long m = a + (b - a) / 2;
+        int l = build_nodes(a, m, nodes);
+        int r = build_nodes(m, b, nodes);
+        nodes[idx].left = l;
+        nodes[idx].right = r;
+        nodes[idx].height = 1 + std::max(nodes[l].height, nodes[r].height);
+    }
+    return idx;
+}
+
+struct ntt_split_tree {
+    long a = 0, b = 0;
+    std::vector<TreeNode> nodes;
+    uint64_t *arena = nullptr;
+    size_t arena_words = 0;
+    LeafDesc *d_leaves = nullptr;
+    uint32_t n_leaves = 0;
+    MulDesc *d_desc = nullptr;
+    std::vector<BatchPlan> levels;
+    std::vector<size_t> level_off;   /* offset of each level's descriptors in d_desc */
+    Workspace ws;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t n_products = 0;
+};
+
+static void tree_free(ntt_split_tree *t)
+{
+    if (!t) return;
+    if (t->exec) (void)hipGraphExecDestroy(t->exec);
+    if (t->graph) (void)hipGraphDestroy(t->graph);
+    if (t->arena) (void)hipFree(t->arena);
+    if (t->d_leaves) (void)hipFree(t->d_leaves);
+    if (t->d_desc) (void)hipFree(t->d_desc);
+    t->ws.release();
+    delete t;
+}
+
+/* Enqueue leaves and every level on `st` (also used under graph capture). */
+static int tree_enqueue(ntt_split_tree *t, hipStream_t st)
+{
+    if (t->n_leaves) {
+        hipLaunchKernelGGL(k_leaves, dim3((t->n_leaves + 255) / 256), dim3(256), 0, st, t->d_leaves,
+                           t->n_leaves, t->arena);
+        HIP_TRY(hipGetLastError());
+    }
+    for (size_t h = 0; h < t->levels.size(); h++) {
+        if (t->levels[h].sorted.empty()) continue;
+        int rc = enqueue_batch(t->ws, t->levels[h], t->d_desc + t->level_off[h], st);
+        if (rc) return rc;
+    }
+    return NTT_OK;
+}
+
+extern "C" {
+
+void ntt_split_tree_destroy(ntt_split_tree *t)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (t && g.init) (void)hipStreamSynchronize(g.stream);
+    tree_free(t);
+}
+
+int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (!out) return fail(NTT_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    if (a < 1 || (b > a && b - 1 > 0xffffffffL)) return fail(NTT_ERR_ARG, "range [%ld, %ld) unsupported", a, b);
+    ntt_split_tree *t = new ntt_split_tree();
+    t->a = a;
+    t->b = b;
+    if (b <= a) {
+        TreeNode nd;
+        nd.a = a;
+        nd.b = b;
+        t->nodes.push_back(nd);
+    } else {
+        build_nodes(a, b, t->nodes);
+    }
+    /* capacities and arena offsets */
+    size_t off = 0;
+    int H = 0;
+    for (auto &nd : t->nodes) {
+        nd.capB = cap_limbs_B(nd.a, nd.b);
+        nd.capP = nd.capB + 1;
+        if (nd.left < 0) {   /* leaf: values fit LEAF_LIMBS */
+            nd.capB = std::max<uint32_t>(std::min<uint32_t>(nd.capB, LEAF_LIMBS), 1);
+            nd.capP = std::max<uint32_t>(std::min<uint32_t>(nd.capP, LEAF_LIMBS), 1);
+        }
+        nd.offP = off;
+        off += nd.capP;
+        nd.offB = off;
+        off += nd.capB;
+        H = std::max(H, nd.height);
+    }
+    t->arena_words = off;
+    int rc = NTT_OK;
+    if (hipMalloc((void **)&t->arena, off * 8) != hipSuccess) rc = fail(NTT_ERR_NOMEM, "tree arena (%zu limbs)", off);
+    /* leaves */
+    std::vector<LeafDesc> leaves;
+    for (auto &nd : t->nodes)
+        if (nd.left < 0)
+            leaves.push_back({(uint64_t)nd.a, (uint64_t)nd.b, nd.offP, nd.offB, nd.capP, nd.capB});
+    t->n_leaves = (uint32_t)leaves.size();
+    if (!rc && hipMalloc((void **)&t->d_leaves, leaves.size() * sizeof(LeafDesc)) != hipSuccess)
+        rc = fail(NTT_ERR_NOMEM, "leaf descriptors");
+    if (!rc && hipMemcpy(t->d_leaves, leaves.data(), leaves.size() * sizeof(LeafDesc), hipMemcpyHostToDevice) !=
+                   hipSuccess)
+        rc = fail(NTT_ERR_HIP, "leaf descriptor upload");
+    /* one batch per height: B = B_L*B_R and P = P_L*B_R + P_R */
+    std::vector<std::vector<ntt_mul_desc>> per(H + 1);
+    for (auto &nd : t->nodes) {
+        if (nd.left < 0) continue;
+        const TreeNode &L = t->nodes[nd.left], &R = t->nodes[nd.right];
+        uint64_t *A = t->arena;
+        ntt_mul_desc dB{A + L.offB, A + R.offB, A + nd.offB, nullptr, L.capB, R.capB, 0, nd.capB};
+        ntt_mul_desc dP{A + L.offP, A + R.offB, A + nd.offP, A + R.offP, L.capP, R.capB, R.capP, nd.capP};
+        per[nd.height].push_back(dB);
+        per[nd.height].push_back(dP);
+    }
+    t->levels.resize(H + 1);
+    t->level_off.resize(H + 1);
+    size_t ndesc = 0, res_words = 1, scratch = 64;
+    for (int h = 1; h <= H && !rc; h++) {
+        t->level_off[h] = ndesc;
+        rc = plan_batch(per[h].data(), per[h].size(), t->levels[h]);
+        ndesc += per[h].size();
+        res_words = std::max(res_words, t->levels[h].res_words);
+        scratch = std::max(scratch, t->levels[h].scratch);
+    }
+    t->n_products = ndesc;
+    if (!rc && ndesc) {
+        if (hipMalloc((void **)&t->d_desc, ndesc * sizeof(MulDesc)) != hipSuccess)
+            rc = fail(NTT_ERR_NOMEM, "tree descriptors");
+        for (int h = 1; h <= H && !rc; h++)
+            if (!t->levels[h].sorted.empty() &&
+                hipMemcpy(t->d_desc + t->level_off[h], t->levels[h].sorted.data(),
+                          t->levels[h].sorted.size() * sizeof(MulDesc), hipMemcpyHostToDevice) != hipSuccess)
+                rc = fail(NTT_ERR_HIP, "tree descriptor upload");
+    }
+    if (!rc) rc = ws_reserve(t->ws, res_words, scratch);
+    if (!rc && use_graph) {
+        if (hipStreamSynchronize(g.stream) != hipSuccess) rc = fail(NTT_ERR_HIP, "stream sync");
+        int saved = g.timing;
+        g.timing = 0;
+        hipError_t e = rc ? hipSuccess : hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal);
+        if (!rc && e != hipSuccess) rc = fail(NTT_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+        if (!rc) {
+            rc = tree_enqueue(t, g.stream);
+            e = hipStreamEndCapture(g.stream, &t->graph);
+            if (!rc && e != hipSuccess) rc = fail(NTT_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+        }
+        g.timing = saved;
+        if (!rc && hipGraphInstantiate(&t->exec, t->graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = fail(NTT_ERR_HIP, "hipGraphInstantiate failed");
+    }
+    if (rc) {
+        tree_free(t);
+        return rc;
+    }
+    *out = t;
+    return NTT_OK;
+}
+
+/* Evaluate the tree (graph replay, or eager launches) and wait. */
+int ntt_split_tree_run(ntt_split_tree *t)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    if (!t) return fail(NTT_ERR_ARG, "null tree");
+    g.dispatches += t->n_products;
+    if (t->exec) {
+        HIP_TRY(hipGraphLaunch(t->exec, g.stream));
+    } else {
+        int rc = tree_enqueue(t, g.stream);
+        if (rc) return rc;
+    }
+    return sync_check();
+}
+
+int ntt_split_tree_info(ntt_split_tree *t, size_t *n_products, int *n_levels, size_t *n_leaves, size_t *arena_bytes)
+{
+    if (!t) return fail(NTT_ERR_ARG, "null tree");
+    if (n_products) *n_products = t->n_products;
+    if (n_levels) *n_levels = (int)t->levels.size() - 1;
+    if (n_leaves) *n_leaves = t->n_leaves;
+    if (arena_bytes) *arena_bytes = t->arena_words * 8;
+    return NTT_OK;
+}
+
+/* Root values on the device (capacities, not trimmed lengths). */
+int ntt_split_tree_device_result(ntt_split_tree *t, const uint64_t **dP, size_t *capP, const uint64_t **dB,
+                                 size_t *capB)
+{
+    if (!t) return fail(NTT_ERR_ARG, "null tree");
+    const TreeNode &r = t->nodes[0];
+    if (dP) *dP = t->arena + r.offP;
+    if (capP) *capP = r.capP;
+    if (dB) *dB = t->arena + r.offB;
+    if (capB) *capB = r.capB;
+    return NTT_OK;
+}
+
+/* Copy the root P and B to host buffers; lengths returned trimmed. */
+int ntt_split_tree_result(ntt_split_tree *t, uint64_t *P, size_t capP, size_t *nP, uint64_t *B, size_t capB,
+                          size_t *nB)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (!t) return fail(NTT_ERR_ARG, "null tree");
+    const TreeNode &r = t->nodes[0];
+    if (capP < r.capP || capB < r.capB)
+        return fail(NTT_ERR_ARG, "result buffers too small (need %u and %u limbs)", r.capP, r.capB);
+    HIP_TRY(hipMemcpy(P, t->arena + r.offP, (size_t)r.capP * 8, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(B, t->arena + r.offB, (size_t)r.capB * 8, hipMemcpyDeviceToHost));
+    size_t np = r.capP, nb = r.capB;
+    while (np && P[np - 1] == 0) np--;
+    while (nb && B[nb - 1] == 0) nb--;
+    if (nP) *nP = np;
+    if (nB) *nB = nb;
+    return NTT_OK;
+}
+
+int ntt_split_tree_root_caps(ntt_split_tree *t, size_t *capP, size_t *capB)
+{
+    if (!t) return fail(NTT_ERR_ARG, "null tree");
+    if (capP) *capP = t->nodes[0].capP;
+    if (capB) *capB = t->nodes[0].capB;
+    return NTT_OK;
+}
+
+}  // extern "C"

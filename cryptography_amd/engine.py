@@ -17,6 +17,13 @@ STATUS = {0: "NTT_OK", 1: "NTT_ERR_NOT_INIT", 2: "NTT_ERR_TOO_LARGE", 3: "NTT_ER
           4: "NTT_ERR_ARG", 5: "NTT_ERR_INTEGRITY", 6: "NTT_ERR_NOMEM"}
 
 
+class MulDesc(ctypes.Structure):
+    """ntt_mul_desc: c[0..n_out) = a*b + add, device pointers."""
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p),
+                ("add", ctypes.c_void_p), ("na", ctypes.c_uint32), ("nb", ctypes.c_uint32),
+                ("nadd", ctypes.c_uint32), ("n_out", ctypes.c_uint32)]
+
+
 class NttError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{STATUS.get(code, code)}: {msg}")
@@ -54,6 +61,12 @@ def lib():
     L.ntt_memcpy_h2d.argtypes = [vp, vp, sz]
     L.ntt_memcpy_d2h.argtypes = [vp, vp, sz]
     L.ntt_engine_read_twiddles.argtypes = [_u64p, sz]
+    L.ntt_engine_kernel_times.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    L.ntt_mul_batch_device.argtypes = [ctypes.POINTER(MulDesc), sz]
+    L.ntt_mul_batch_device_async.argtypes = [ctypes.POINTER(MulDesc), sz]
+    L.ntt_batch_graph_create.argtypes = [ctypes.POINTER(MulDesc), sz, ctypes.POINTER(ctypes.c_void_p)]
+    L.ntt_batch_graph_launch.argtypes = [ctypes.c_void_p]
+    L.ntt_batch_graph_destroy.argtypes = [ctypes.c_void_p]
     _lib = L
     return L
 
@@ -146,3 +159,124 @@ def mul_device(dc, da, na, db, nb, sync=True):
 def sqr_device(dc, da, na, sync=True):
     f = lib().ntt_sqr_device if sync else lib().ntt_sqr_device_async
     check(f(dc.ptr, da.ptr, na))
+
+
+class DevicePool:
+    """One device allocation carved into u64 sub-buffers (offsets in words)."""
+
+    def __init__(self, n_words):
+        self.buf = DeviceBuffer(n_words)
+        self.used = 0
+
+    def take(self, n):
+        off = self.used
+        self.used += max(1, int(n))
+        assert self.used <= self.buf.n, "pool exhausted"
+        return off
+
+    def ptr(self, off):
+        return self.buf.ptr + 8 * off
+
+    def upload(self, off, a):
+        a = _as_u64(a)
+        if len(a):
+            check(lib().ntt_memcpy_h2d(self.ptr(off), a.ctypes.data, len(a) * 8))
+
+    def download(self, off, n):
+        out = np.zeros(n, dtype=np.uint64)
+        if n:
+            check(lib().ntt_memcpy_d2h(out.ctypes.data, self.ptr(off), n * 8))
+        return out
+
+    def close(self):
+        self.buf.close()
+
+
+def make_descs(items):
+    """items: (a_ptr, na, b_ptr, nb, c_ptr, n_out, add_ptr, nadd) tuples -> ctypes array."""
+    arr = (MulDesc * len(items))()
+    for i, (pa, na, pb, nb, pc, n_out, padd, nadd) in enumerate(items):
+        arr[i] = MulDesc(pa, pb, pc, padd, na, nb, nadd, n_out)
+    return arr
+
+
+def mul_batch_device(descs, sync=True):
+    f = lib().ntt_mul_batch_device if sync else lib().ntt_mul_batch_device_async
+    check(f(descs, len(descs)))
+
+
+class BatchGraph:
+    """A batch captured into a hipGraph; launch() replays it."""
+
+    def __init__(self, descs):
+        h = ctypes.c_void_p()
+        check(lib().ntt_batch_graph_create(descs, len(descs), ctypes.byref(h)))
+        self.h = h
+
+    def launch(self):
+        check(lib().ntt_batch_graph_launch(self.h))
+
+    def close(self):
+        if self.h:
+            lib().ntt_batch_graph_destroy(self.h)
+            self.h = None
+
+
+def sync():
+    check(lib().ntt_engine_sync())
+
+
+def split_tree(a, b, use_graph=True, runs=1):
+    """(P, B) of sum_{k=a}^{b-1} 1/k! from the level-batched device tree."""
+    L = lib()
+    init()
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    L.ntt_split_tree_create.argtypes = [ctypes.c_long, ctypes.c_long, ctypes.c_int, ctypes.POINTER(vp)]
+    L.ntt_split_tree_run.argtypes = [vp]
+    L.ntt_split_tree_root_caps.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)]
+    L.ntt_split_tree_result.argtypes = [vp, _u64p, sz, ctypes.POINTER(sz), _u64p, sz, ctypes.POINTER(sz)]
+    L.ntt_split_tree_destroy.argtypes = [vp]
+    h = vp()
+    check(L.ntt_split_tree_create(a, b, 1 if use_graph else 0, ctypes.byref(h)))
+    try:
+        for _ in range(runs):
+            check(L.ntt_split_tree_run(h))
+        cp, cb = sz(), sz()
+        check(L.ntt_split_tree_root_caps(h, ctypes.byref(cp), ctypes.byref(cb)))
+        P = np.zeros(cp.value, np.uint64)
+        B = np.zeros(cb.value, np.uint64)
+        n_p, n_b = sz(), sz()
+        check(L.ntt_split_tree_result(h, _p(P), cp.value, ctypes.byref(n_p), _p(B), cb.value, ctypes.byref(n_b)))
+        return P[:n_p.value].copy(), B[:n_b.value].copy()
+    finally:
+        L.ntt_split_tree_destroy(h)
+
+
+class BigIntC(ctypes.Structure):
+    """BigInt of include/bigint.h."""
+    _fields_ = [("limbs", _u64p), ("n_limbs", ctypes.c_int), ("cap", ctypes.c_int), ("managed", ctypes.c_int)]
+
+
+class SplitResultC(ctypes.Structure):
+    _fields_ = [("P", BigIntC), ("B", BigIntC)]
+
+
+def _bigint_to_np(b):
+    return np.ctypeslib.as_array(b.limbs, shape=(b.n_limbs,)).copy() if b.n_limbs else np.zeros(0, np.uint64)
+
+
+def binary_split_ref_api(a, b, batched=False):
+    """(P, B) through the reference-compatible binary_split / binary_split_batched."""
+    L = lib()
+    init()
+    L.binary_split.argtypes = [ctypes.POINTER(SplitResultC), ctypes.c_long, ctypes.c_long, ctypes.c_void_p]
+    L.binary_split_batched.argtypes = [ctypes.POINTER(SplitResultC), ctypes.c_long, ctypes.c_long]
+    L.split_free.argtypes = [ctypes.POINTER(SplitResultC)]
+    sr = SplitResultC()
+    if batched:
+        L.binary_split_batched(ctypes.byref(sr), a, b)
+    else:
+        L.binary_split(ctypes.byref(sr), a, b, None)
+    P, B = _bigint_to_np(sr.P), _bigint_to_np(sr.B)
+    L.split_free(ctypes.byref(sr))
+    return P, B

@@ -61,6 +61,32 @@ int ntt_mul_device_async(uint64_t *d_c, const uint64_t *d_a, size_t na, const ui
 int ntt_sqr_device_async(uint64_t *d_c, const uint64_t *d_a, size_t na);
 int ntt_engine_sync(void);
 
+/* ---- batches ---------------------------------------------------------------
+ * One product of a batch: c[0..n_out) = a*b + add (add may be NULL, nadd 0).
+ * All pointers are device pointers; n_out >= na + nb and >= nadd, and the
+ * result must fit n_out limbs (else NTT_ERR_INTEGRITY at the next sync).
+ * No counterpart in the reference (every reference multiply is one call):
+ * this is what the level-batched product tree and the serving loop use. */
+typedef struct {
+    const uint64_t *a, *b;
+    uint64_t *c;
+    const uint64_t *add;
+    uint32_t na, nb, nadd, n_out;
+} ntt_mul_desc;
+
+/* Products are grouped by algorithm and transform length (schoolbook class,
+ * one class per NTT length); each class runs as one launch per kernel stage. */
+int ntt_mul_batch_device(const ntt_mul_desc *descs, size_t n);
+int ntt_mul_batch_device_async(const ntt_mul_desc *descs, size_t n);
+
+/* The same batch captured once into a hipGraph with its own workspace, then
+ * replayed with one hipGraphLaunch per call (pointers and sizes are fixed at
+ * capture; operand contents may change between launches). */
+typedef struct ntt_batch_graph ntt_batch_graph;
+int ntt_batch_graph_create(const ntt_mul_desc *descs, size_t n, ntt_batch_graph **out);
+int ntt_batch_graph_launch(ntt_batch_graph *g);   /* async; pair with ntt_engine_sync() */
+void ntt_batch_graph_destroy(ntt_batch_graph *g);
+
 /* Schoolbook / NTT crossover in limbs (both operands at or below it use the
  * GPU schoolbook kernel).  Cf. multiply.h:23-31. */
 int ntt_engine_get_threshold(void);

@@ -1,0 +1,116 @@
+"""GPU parity of the batched multiply and its hipGraph replay (through the C ABI).
+
+Each product of a batch must equal the same product computed alone, and GMP.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import _oracle as O
+import cryptography_amd as E
+from cryptography_amd import engine as EN
+
+pytestmark = pytest.mark.gpu
+
+
+def build_batch(pool, specs, seed):
+    """specs: list of (na, nb, nadd).  Returns descriptors and host operands."""
+    items, host = [], []
+    for i, (na, nb, nadd) in enumerate(specs):
+        a = O.splitmix(seed * 1000 + 2 * i, na)
+        b = O.splitmix(seed * 1000 + 2 * i + 1, nb)
+        add = O.splitmix(seed * 7777 + i, nadd) if nadd else np.zeros(0, np.uint64)
+        n_out = max(na + nb, nadd) + (1 if nadd else 0)
+        oa, ob, oc = pool.take(na), pool.take(nb), pool.take(n_out)
+        od = pool.take(nadd) if nadd else None
+        pool.upload(oa, a)
+        pool.upload(ob, b)
+        if nadd:
+            pool.upload(od, add)
+        items.append((pool.ptr(oa), na, pool.ptr(ob), nb, pool.ptr(oc), n_out,
+                      pool.ptr(od) if nadd else None, nadd))
+        host.append((a, b, add, oc, n_out))
+    return EN.make_descs(items), host
+
+
+def expected(a, b, add, n_out):
+    v = O.limbs_to_int(a) * O.limbs_to_int(b) + (O.limbs_to_int(add) if len(add) else 0)
+    return O.int_to_limbs(v, n_out)
+
+
+MIXED = [(1, 1, 0), (3, 5, 0), (0, 9, 0), (64, 64, 0), (64, 64, 70), (65, 3, 0), (100, 100, 0),
+         (100, 100, 100), (500, 700, 0), (513, 511, 300), (1000, 1000, 0), (1000, 999, 1000),
+         (4096, 4000, 0), (9000, 9000, 9000), (20000, 1, 0), (17, 30000, 0), (5, 5, 12)]
+
+
+def test_batch_mixed_sizes_with_addend():
+    pool = EN.DevicePool(400000)
+    try:
+        descs, host = build_batch(pool, MIXED, seed=3)
+        EN.mul_batch_device(descs)
+        for (a, b, add, oc, n_out) in host:
+            assert np.array_equal(pool.download(oc, n_out), expected(a, b, add, n_out))
+    finally:
+        pool.close()
+
+
+def test_batch_equals_alone_and_gmp():
+    rng = random.Random(4)
+    specs = [(rng.randrange(1, 3000), rng.randrange(1, 3000), 0) for _ in range(40)]
+    pool = EN.DevicePool(600000)
+    try:
+        descs, host = build_batch(pool, specs, seed=4)
+        EN.mul_batch_device(descs)
+        for (a, b, add, oc, n_out) in host:
+            got = pool.download(oc, n_out)
+            assert np.array_equal(got, O.gmp_mul(a, b))
+            assert np.array_equal(got, E.mul(a, b))          # the same product alone
+    finally:
+        pool.close()
+
+
+def test_batch_graph_replays_match_eager():
+    specs = [(7, 9, 0), (64, 64, 64), (300, 200, 0), (2000, 2100, 0), (2000, 2100, 50), (40, 5000, 0)]
+    pool = EN.DevicePool(200000)
+    try:
+        descs, host = build_batch(pool, specs, seed=5)
+        EN.mul_batch_device(descs)
+        eager = [pool.download(oc, n_out) for (_, _, _, oc, n_out) in host]
+        for (a, b, add, oc, n_out), e in zip(host, eager):
+            assert np.array_equal(e, expected(a, b, add, n_out))
+        g = EN.BatchGraph(descs)
+        try:
+            for rep in range(4):
+                # clobber the outputs between replays with eager work
+                for (_, _, _, oc, n_out) in host:
+                    pool.upload(oc, np.full(n_out, 0xDEADBEEF, np.uint64))
+                x = O.splitmix(rep, 100)
+                assert np.array_equal(E.mul(x, x), O.gmp_mul(x, x))
+                g.launch()
+                EN.sync()
+                for (_, _, _, oc, n_out), e in zip(host, eager):
+                    assert np.array_equal(pool.download(oc, n_out), e), f"replay {rep}"
+        finally:
+            g.close()
+    finally:
+        pool.close()
+
+
+def test_batch_overflow_is_reported():
+    # n_out too small for the addend carry: integrity error at sync
+    pool = EN.DevicePool(1000)
+    try:
+        a = np.full(4, 0xFFFFFFFFFFFFFFFF, np.uint64)
+        oa, oc, od = pool.take(4), pool.take(8), pool.take(8)
+        pool.upload(oa, a)
+        pool.upload(od, np.full(8, 0xFFFFFFFFFFFFFFFF, np.uint64))
+        descs = EN.make_descs([(pool.ptr(oa), 4, pool.ptr(oa), 4, pool.ptr(oc), 8, pool.ptr(od), 8)])
+        with pytest.raises(E.NttError) as ei:
+            EN.mul_batch_device(descs)
+        assert ei.value.code == 5
+        # the engine keeps working afterwards
+        x = O.splitmix(9, 50)
+        assert np.array_equal(E.mul(x, x), O.gmp_mul(x, x))
+    finally:
+        pool.close()
