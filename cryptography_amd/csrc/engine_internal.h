@@ -23,6 +23,7 @@ struct Engine {
     int device = 0;
     hipStream_t stream = nullptr;
     TwPair *tw = nullptr;
+    TwPair *small_tw = nullptr;     /* w_4096^e per prime: stage twiddles */
     int *flag = nullptr;
     MulDesc *d_desc = nullptr;      /* descriptor array of the last batch call */
     size_t d_desc_n = 0;

@@ -102,6 +102,8 @@ int engine_init(int device)
     hipLaunchKernelGGL(k_twiddle_gen, dim3(NTT_TW_SIZE / 256, NTT_NPRIMES), dim3(256), 0, g.stream, g.tw,
                        r2[0], r2[1], r2[2]);
     HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMalloc((void **)&g.small_tw, sizeof(TwPair) * NTT_NPRIMES * (FT_TILE / 2)));
+    HIP_TRY(launch_small_twiddles(g.tw, g.small_tw, g.stream));
     HIP_TRY(hipStreamSynchronize(g.stream));
     /* Garner constants (garner.hip:105-132 restated for three primes) */
     const uint64_t p0 = prime(0), p1 = prime(1), p2 = prime(2);
@@ -120,6 +122,7 @@ void engine_teardown()
     if (!g.init) return;
     (void)hipStreamSynchronize(g.stream);
     (void)hipFree(g.tw);
+    (void)hipFree(g.small_tw);
     (void)hipFree(g.flag);
     g_ws.release();
     if (g.d_desc) (void)hipFree(g.d_desc);
@@ -142,18 +145,24 @@ int log_len(size_t na, size_t nb)
     return l;
 }
 
+/* Pass sizes of a length-2^logL transform: as few passes as possible with at
+ * most 2^FT_MAX_LOGS points each, balanced, the largest last (the last pass
+ * is contiguous, the earlier ones read columns). */
 PassPlan plan_passes(int logL)
 {
     PassPlan pp{};
-    if (logL <= 12) {
-        pp.npass = 1;
-        pp.logS[0] = logL;
-        return pp;
-    }
-    pp.npass = (logL + 8) / 9;
+    pp.npass = (logL + FT_MAX_LOGS - 1) / FT_MAX_LOGS;
+    if (pp.npass < 1) pp.npass = 1;
     int base = logL / pp.npass, extra = logL % pp.npass;
-    for (int i = 0; i < pp.npass; i++) pp.logS[i] = base + (i < extra ? 1 : 0);
+    for (int i = 0; i < pp.npass; i++) pp.logS[i] = base + (i >= pp.npass - extra ? 1 : 0);
     return pp;
+}
+
+/* Residue row length of a class: nprod * L rounded up to whole tiles. */
+size_t row_len(int logL, uint32_t nprod)
+{
+    size_t len = ((size_t)nprod) << logL;
+    return (len + FT_TILE - 1) / FT_TILE * FT_TILE;
 }
 
 __global__ void k_fill_zero(uint64_t *p, size_t n)
@@ -192,7 +201,7 @@ void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t
 {
     uint32_t thr, blk;
     carry_geom(max_nout, thr, blk);
-    res_words = (size_t)(square ? 1 : 2) * NTT_NPRIMES * nprod * ((size_t)1 << logL);
+    res_words = (size_t)(square ? 1 : 2) * NTT_NPRIMES * row_len(logL, nprod);
     scratch = (size_t)nprod * (thr + blk) + 64;
 }
 
@@ -202,57 +211,57 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
                       uint32_t max_nout, int square, hipStream_t st)
 {
     const size_t L = (size_t)1 << logL;
-    const size_t len = L * nprod;
+    const size_t len = row_len(logL, nprod);
     const int nops = square ? 1 : 2;
     size_t need_res, need_scr;
     class_needs(logL, nprod, max_nout, square, need_res, need_scr);
     if (ws.res_words < need_res || ws.scratch_bytes < need_scr)
         return fail(NTT_ERR_ARG, "workspace too small for the class (internal)");
     const PassPlan pp = plan_passes(logL);
-    const uint32_t tile_log = logL < 12 ? logL : 12;
-    const size_t ntiles = len >> tile_log;
-    const size_t lds = (size_t)8 << tile_log;
+    const size_t ntiles = len / FT_TILE;
 
     PassArgs pa{};
     pa.res = ws.res;
     pa.tw = g.tw;
+    pa.small_tw = g.small_tw;
     pa.desc = d_desc;
     pa.one = one;
     pa.len = len;
+    pa.nprod = nprod;
     pa.logL = logL;
-    uint32_t done = 0;
-    for (int i = 0; i < pp.npass; i++) {
-        pa.logS = pp.logS[i];
-        pa.logT = logL - done - pa.logS;
-        pa.logC = pa.logT < tile_log - pa.logS ? pa.logT : tile_log - pa.logS;
-        pa.logG = tile_log - pa.logS - pa.logC;
-        pa.first = i == 0;
-        hipLaunchKernelGGL(k_fwd_pass, dim3(ntiles, NTT_NPRIMES, nops), dim3(256), lds, st, pa);
-        HIP_TRY(mark("k_fwd_pass"));
-        done += pa.logS;
-    }
-    PointwiseArgs pw{};
-    pw.res = ws.res;
-    pw.len = len;
-    pw.square = square;
+    pa.square = square;
     for (int i = 0; i < NTT_NPRIMES; i++) {
         uint64_t p = prime(i);
         uint64_t r = (uint64_t)(((u128)1 << 64) % p);
-        pw.scale[i] = h_mulmod(r, h_inv(L % p, p), p);
-        pw.scale_p[i] = h_shoup(pw.scale[i], p);
+        pa.scale[i] = h_mulmod(r, h_inv(L % p, p), p);
+        pa.scale_p[i] = h_shoup(pa.scale[i], p);
     }
-    size_t pwb = len / 256 < 8192 ? (len + 255) / 256 : 8192;
-    hipLaunchKernelGGL(k_pointwise, dim3(pwb, NTT_NPRIMES), dim3(256), 0, st, pw);
-    HIP_TRY(mark("k_pointwise"));
-    for (int i = pp.npass - 1; i >= 0; i--) {
+    /* geometry of pass i: S = 2^logS columns of stride T inside blocks of S*T */
+    auto geom = [&](int i, uint32_t done) {
+        uint32_t logS = pp.logS[i];
+        pa.logT = logL - done - logS;
+        uint32_t room = 12 - logS;                   /* log2(FT_TILE / S) */
+        pa.logC = pa.logT < room ? pa.logT : room;
+        pa.logG = room - pa.logC;
+    };
+    uint32_t done = 0;
+    for (int i = 0; i < pp.npass - 1; i++) {
+        geom(i, done);
+        pa.first = i == 0;
+        HIP_TRY(launch_fast_pass(pp.logS[i], 0, pa, ntiles, nops, st));
+        HIP_TRY(mark("k_fast_pass<fwd>"));
+        done += pp.logS[i];
+    }
+    geom(pp.npass - 1, done);
+    pa.first = pp.npass == 1;
+    HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], 2, pa, ntiles, nops, st));
+    HIP_TRY(mark("k_fast_pass<mid>"));
+    for (int i = pp.npass - 2; i >= 0; i--) {
         done -= pp.logS[i];
-        pa.logS = pp.logS[i];
-        pa.logT = logL - done - pa.logS;
-        pa.logC = pa.logT < tile_log - pa.logS ? pa.logT : tile_log - pa.logS;
-        pa.logG = tile_log - pa.logS - pa.logC;
+        geom(i, done);
         pa.first = 0;
-        hipLaunchKernelGGL(k_inv_pass, dim3(ntiles, NTT_NPRIMES, 1), dim3(256), lds, st, pa);
-        HIP_TRY(mark("k_inv_pass"));
+        HIP_TRY(launch_fast_pass(pp.logS[i], 1, pa, ntiles, nops, st));
+        HIP_TRY(mark("k_fast_pass<inv>"));
     }
     g.ntt_mark = g.n_marks - 1;
     GarnerArgs ga{};

@@ -1,0 +1,377 @@
+/*
+ * ntt_fast.hip — the transform passes of the CRT-NTT multiply, register-blocked.
+ *
+ * Same four-step decomposition as the generic passes in ntt_kernels.hip (pass
+ * i runs S-point sub-transforms down the columns of an S x T view of the
+ * current block, then multiplies by the cross twiddle w_N^(r*bitrev(m))), with
+ * three changes that matter on gfx950:
+ *   - the S-point sub-transform runs in radix-8 rounds: each thread takes 8
+ *     elements from LDS, does three butterfly stages in registers and writes
+ *     them back, so a 512-point transform costs 3 LDS round trips and 3
+ *     barriers instead of 9;
+ *   - the S/2 stage twiddles (and their inverses) are staged once per block in
+ *     LDS as Shoup pairs;
+ *   - the last forward pass of both operands, the pointwise product (with the
+ *     1/L scale) and the first inverse pass are one kernel (k_mid_pass): both
+ *     spectra of a tile are formed, multiplied and transformed back without
+ *     leaving the chip.
+ * A tile is 4096 residues (32 KB of LDS) worked on by 512 threads; rows of a
+ * size class are padded to a multiple of 4096.
+ *
+ * Reference counterparts: ntt/lib/ntt_kernel.hip:625-675 (stockham_kernel),
+ * :739-778 (stockham_kernel_had: Hadamard fused into the inverse's first pass),
+ * :880-918 (xtranspose_fwd/inv: cross twiddle).
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "modarith.h"
+#include "ntt_kernels.h"
+
+using namespace nttc;
+
+#define FT_LOG 12                 /* tile: 4096 residues */
+#define FT_SIZE (1u << FT_LOG)
+#define FTPB 512                  /* threads per block */
+#define FT_PER (FT_SIZE / FTPB)   /* residues per thread */
+
+/* Shoup pairs w_4096^e, e < 2048, per prime: the stage twiddles of every
+ * sub-transform of up to 4096 points (w_S^x = w_4096^(x*4096/S)). */
+__global__ void __launch_bounds__(256) k_small_twiddles(const TwPair *tw, TwPair *small)
+{
+    /* w_4096^e = W^(e * 2^18) for the 2^30 root W: entry e*2^3 of the hi table */
+    uint32_t e = blockIdx.x * 256 + threadIdx.x;
+    uint32_t pi = blockIdx.y;
+    if (e >= FT_SIZE / 2) return;
+    small[pi * (FT_SIZE / 2) + e] = tw[(size_t)pi * 2 * NTT_TW_SIZE + NTT_TW_SIZE + (e << 3)];
+}
+
+__device__ __forceinline__ uint32_t brev_n(uint32_t x, int bits)
+{
+    return bits ? (__brev(x) >> (32 - bits)) : 0;
+}
+
+template <int PI>
+__device__ __forceinline__ uint64_t xtw(uint64_t x, const TwPair *tw, uint32_t E)
+{
+    TwPair lo = tw[E & (NTT_TW_SIZE - 1)];
+    TwPair hi = tw[NTT_TW_SIZE + (E >> NTT_TW_LOG)];
+    x = shoup_lazy<PI>(x, lo.w, lo.wp);
+    return shoup_lazy<PI>(x, hi.w, hi.wp);
+}
+
+/* Tile geometry: G blocks x S rows x C columns, LDS index ((g*S + m) << logC) + c. */
+struct FGeom {
+    uint32_t logT, logC, logG;
+};
+
+__device__ __forceinline__ size_t ft_addr(int LOGS, const FGeom &g, uint32_t t, uint32_t q, uint32_t &m, uint32_t &r)
+{
+    uint32_t c = q & ((1u << g.logC) - 1);
+    m = (q >> g.logC) & ((1u << LOGS) - 1);
+    uint32_t gg = q >> (g.logC + LOGS);
+    uint32_t colTiles = g.logT - g.logC;
+    size_t b0 = ((size_t)(t >> colTiles)) << g.logG;
+    uint32_t col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+    r = col0 + c;
+    return ((b0 + gg) << (LOGS + g.logT)) + r + ((size_t)m << g.logT);
+}
+
+/* ---- radix-R rounds over the LDS tile -------------------------------------
+ * A DIF round with top stage half-size h = 2^LH and R = 2^RL covers stages
+ * h, h/2, .., h/2^(RL-1): the groups {m0 + i*d}, d = h >> (RL-1),
+ * m0 = blk*2h + j, j < d. */
+
+template <int PI, int LOGS, int LH, int RL>
+__device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint32_t logC)
+{
+    constexpr int R = 1 << RL;
+    constexpr int LD = LH - (RL - 1);
+    constexpr uint32_t d = 1u << LD;
+    constexpr int LNB = LOGS - LH - 1;
+    for (uint32_t gid = threadIdx.x; gid < (FT_SIZE >> RL); gid += FTPB) {
+        uint32_t c = gid & ((1u << logC) - 1);
+        uint32_t u = gid >> logC;
+        uint32_t j = u & (d - 1);
+        uint32_t v = u >> LD;
+        uint32_t blk = v & ((1u << LNB) - 1);
+        uint32_t gg = v >> LNB;
+        uint32_t m0 = (blk << (LH + 1)) + j;
+        uint32_t base = (((gg << LOGS) + m0) << logC) + c;
+        uint64_t x[R];
+#pragma unroll
+        for (int i = 0; i < R; i++) x[i] = lds[base + ((uint32_t)(i << LD) << logC)];
+#pragma unroll
+        for (int s = 0; s < RL; s++) {
+            const int half = R >> (s + 1);          /* pair distance in group index */
+            const int lhs = LH - s;                 /* log2 of this stage's half-size */
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                if (i & half) continue;
+                uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);   /* (m mod hs) */
+                TwPair w = twf[e << (LOGS - 1 - lhs)];
+                dif_bfly<PI>(x[i], x[i + half], w.w, w.wp);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; i++) lds[base + ((uint32_t)(i << LD) << logC)] = x[i];
+    }
+}
+
+template <int PI, int LOGS, int LH, int RL>
+__device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint32_t logC)
+{
+    constexpr int R = 1 << RL;
+    constexpr int LD = LH - (RL - 1);
+    constexpr uint32_t d = 1u << LD;
+    constexpr int LNB = LOGS - LH - 1;
+    for (uint32_t gid = threadIdx.x; gid < (FT_SIZE >> RL); gid += FTPB) {
+        uint32_t c = gid & ((1u << logC) - 1);
+        uint32_t u = gid >> logC;
+        uint32_t j = u & (d - 1);
+        uint32_t v = u >> LD;
+        uint32_t blk = v & ((1u << LNB) - 1);
+        uint32_t gg = v >> LNB;
+        uint32_t m0 = (blk << (LH + 1)) + j;
+        uint32_t base = (((gg << LOGS) + m0) << logC) + c;
+        uint64_t x[R];
+#pragma unroll
+        for (int i = 0; i < R; i++) x[i] = lds[base + ((uint32_t)(i << LD) << logC)];
+#pragma unroll
+        for (int s = RL - 1; s >= 0; s--) {
+            const int half = R >> (s + 1);
+            const int lhs = LH - s;
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                if (i & half) continue;
+                uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
+                TwPair w = twi[e << (LOGS - 1 - lhs)];
+                dit_bfly<PI>(x[i], x[i + half], w.w, w.wp);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; i++) lds[base + ((uint32_t)(i << LD) << logC)] = x[i];
+    }
+}
+
+/* All DIF rounds of an S = 2^LOGS transform, top stage first, radix 8 while
+ * three or more stages remain. */
+template <int PI, int LOGS, int LH>
+__device__ __forceinline__ void dif_all(uint64_t *lds, const TwPair *twf, uint32_t logC)
+{
+    if constexpr (LH >= 0) {
+        constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
+        dif_round<PI, LOGS, LH, RL>(lds, twf, logC);
+        __syncthreads();
+        dif_all<PI, LOGS, LH - RL>(lds, twf, logC);
+    }
+}
+
+/* DIT rounds in the mirror order (lowest stages first). */
+template <int PI, int LOGS, int LH>
+__device__ __forceinline__ void dit_all(uint64_t *lds, const TwPair *twi, uint32_t logC)
+{
+    if constexpr (LH >= 0) {
+        constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
+        dit_all<PI, LOGS, LH - RL>(lds, twi, logC);
+        dit_round<PI, LOGS, LH, RL>(lds, twi, logC);
+        __syncthreads();
+    }
+}
+
+/* Stage twiddles of an S-point transform into LDS: twf[e] = w_S^e,
+ * twi[e] = w_S^-e, e < S/2 (S <= 1024), or point at the global table. */
+template <int PI, int LOGS>
+__device__ __forceinline__ void stage_twiddles(const TwPair *small_all, TwPair *twf, TwPair *twi)
+{
+    const TwPair *sm = small_all + PI * (FT_SIZE / 2);
+    constexpr uint32_t half = 1u << (LOGS - 1);
+    constexpr int ST = FT_LOG - LOGS;      /* w_S^e = w_4096^(e << ST) */
+    for (uint32_t e = threadIdx.x; e < half; e += FTPB) {
+        twf[e] = sm[e << ST];
+        /* w_S^-e = w_S^(S/2 - e) * w_S^(S/2) = -w_S^(S/2 - e) */
+        if (e == 0) {
+            twi[0] = sm[0];
+        } else {
+            TwPair t = sm[(half - e) << ST];
+            /* partner of p - w is floor(2^64 - w 2^64/p) = 2^64 - 1 - wp (w != 0) */
+            twi[e].w = Prime<PI>::P - t.w;
+            twi[e].wp = ~t.wp;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+
+/* Value of forward-pass input element `a` (global index within the row). */
+__device__ __forceinline__ uint64_t first_in(const PassArgs &pa, int op, size_t a)
+{
+    size_t prod = a >> pa.logL;
+    if (prod >= pa.nprod) return 0;
+    const MulDesc &d = pa.desc ? pa.desc[prod] : pa.one;
+    const uint16_t *s = (const uint16_t *)(op ? d.b : d.a);
+    size_t n16 = 4 * (size_t)(op ? d.nb : d.na);
+    size_t loc = a & (((size_t)1 << pa.logL) - 1);
+    return loc < n16 ? (uint64_t)s[loc] : 0;
+}
+
+template <int PI, int LOGS>
+__device__ void fwd_tile(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds, TwPair *twf)
+{
+    FGeom g{pa.logT, pa.logC, pa.logG};
+    const uint32_t t = blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < FT_PER; k++) {
+        uint32_t q = threadIdx.x + k * FTPB, m, r;
+        size_t a = ft_addr(LOGS, g, t, q, m, r);
+        lds[q] = pa.first ? first_in(pa, op, a) : x[a];
+    }
+    __syncthreads();
+    dif_all<PI, LOGS, LOGS - 1>(lds, twf, g.logC);
+    const uint32_t logN = LOGS + g.logT;
+    const TwPair *tw = pa.tw + (size_t)PI * 2 * NTT_TW_SIZE;
+#pragma unroll
+    for (int k = 0; k < FT_PER; k++) {
+        uint32_t q = threadIdx.x + k * FTPB, m, r;
+        size_t a = ft_addr(LOGS, g, t, q, m, r);
+        uint64_t v = lds[q];
+        if (g.logT) v = xtw<PI>(v, tw, (r * brev_n(m, LOGS)) << (NTT_ROOT_LOG - logN));
+        x[a] = v;
+    }
+}
+
+template <int PI, int LOGS>
+__device__ void inv_tile(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair *twi)
+{
+    FGeom g{pa.logT, pa.logC, pa.logG};
+    const uint32_t t = blockIdx.x;
+    const uint32_t logN = LOGS + g.logT;
+    const TwPair *tw = pa.tw + (size_t)PI * 2 * NTT_TW_SIZE;
+#pragma unroll
+    for (int k = 0; k < FT_PER; k++) {
+        uint32_t q = threadIdx.x + k * FTPB, m, r;
+        size_t a = ft_addr(LOGS, g, t, q, m, r);
+        uint64_t v = x[a];
+        if (g.logT) {
+            uint32_t E = (r * brev_n(m, LOGS)) << (NTT_ROOT_LOG - logN);
+            v = xtw<PI>(v, tw, (0u - E) & ((1u << NTT_ROOT_LOG) - 1));
+        }
+        lds[q] = v;
+    }
+    __syncthreads();
+    dit_all<PI, LOGS, LOGS - 1>(lds, twi, g.logC);
+#pragma unroll
+    for (int k = 0; k < FT_PER; k++) {
+        uint32_t q = threadIdx.x + k * FTPB, m, r;
+        size_t a = ft_addr(LOGS, g, t, q, m, r);
+        x[a] = lds[q];
+    }
+}
+
+/* Last forward pass of a and b, pointwise product (x 2^64/L via Montgomery and
+ * a Shoup scale), first inverse pass; result into a's row.  T = 1 here. */
+template <int PI, int LOGS>
+__device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, uint64_t *lds, TwPair *twf,
+                         TwPair *twi)
+{
+    FGeom g{0, 0, pa.logG};
+    const uint32_t t = blockIdx.x;
+    uint64_t ra[FT_PER];
+#pragma unroll
+    for (int k = 0; k < FT_PER; k++) {
+        uint32_t q = threadIdx.x + k * FTPB, m, r;
+        size_t a = ft_addr(LOGS, g, t, q, m, r);
+        lds[q] = pa.first ? first_in(pa, 0, a) : xa[a];
+    }
+    __syncthreads();
+    dif_all<PI, LOGS, LOGS - 1>(lds, twf, 0);
+#pragma unroll
+    for (int k = 0; k < FT_PER; k++) ra[k] = lds[threadIdx.x + k * FTPB];
+    if (!pa.square) {
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < FT_PER; k++) {
+            uint32_t q = threadIdx.x + k * FTPB, m, r;
+            size_t a = ft_addr(LOGS, g, t, q, m, r);
+            lds[q] = pa.first ? first_in(pa, 1, a) : xb[a];
+        }
+        __syncthreads();
+        dif_all<PI, LOGS, LOGS - 1>(lds, twf, 0);
+    }
+    const uint64_t s = pa.scale[PI], sp = pa.scale_p[PI];
+#pragma unroll
+    for (int k = 0; k < FT_PER; k++) {
+        uint32_t q = threadIdx.x + k * FTPB;
+        uint64_t b = pa.square ? ra[k] : lds[q];
+        lds[q] = shoup_lazy<PI>(mont_mul<PI>(ra[k], b), s, sp);
+    }
+    __syncthreads();
+    dit_all<PI, LOGS, LOGS - 1>(lds, twi, 0);
+#pragma unroll
+    for (int k = 0; k < FT_PER; k++) {
+        uint32_t q = threadIdx.x + k * FTPB, m, r;
+        size_t a = ft_addr(LOGS, g, t, q, m, r);
+        xa[a] = lds[q];
+    }
+}
+
+#define TW_LDS_MAX 512   /* stage-twiddle entries per direction held in LDS */
+
+/* kind 0: forward pass, 1: inverse pass, 2: fused middle pass. */
+template <int LOGS, int KIND>
+__global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
+{
+    __shared__ uint64_t lds[FT_SIZE];
+    __shared__ TwPair twf[TW_LDS_MAX], twi[TW_LDS_MAX];
+    static_assert((1 << (LOGS - 1)) <= TW_LDS_MAX, "stage twiddles exceed LDS table");
+    const uint32_t pi = blockIdx.y, op = blockIdx.z;
+    uint64_t *xa = pa.res + ((size_t)op * NTT_NPRIMES + pi) * pa.len;
+    const uint64_t *xb = pa.res + ((size_t)NTT_NPRIMES + pi) * pa.len;
+    switch (pi) {
+#define CASE_PI(P)                                                                     \
+    case P:                                                                            \
+        stage_twiddles<P, LOGS>(pa.small_tw, twf, twi);                                \
+        __syncthreads();                                                               \
+        if (KIND == 0) fwd_tile<P, LOGS>(pa, xa, op, lds, twf);                        \
+        else if (KIND == 1) inv_tile<P, LOGS>(pa, xa, lds, twi);                       \
+        else mid_tile<P, LOGS>(pa, xa, xb, lds, twf, twi);                             \
+        break;
+        CASE_PI(0)
+        CASE_PI(1)
+    default:
+        stage_twiddles<2, LOGS>(pa.small_tw, twf, twi);
+        __syncthreads();
+        if (KIND == 0) fwd_tile<2, LOGS>(pa, xa, op, lds, twf);
+        else if (KIND == 1) inv_tile<2, LOGS>(pa, xa, lds, twi);
+        else mid_tile<2, LOGS>(pa, xa, xb, lds, twf, twi);
+        break;
+#undef CASE_PI
+    }
+}
+
+/* Host-side launcher: one pass of a class (grid.x tiles, grid.y primes,
+ * grid.z operands for forward passes). */
+hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntiles, int nops, hipStream_t st)
+{
+    dim3 grid((unsigned)ntiles, NTT_NPRIMES, kind == 0 ? nops : 1);
+    dim3 block(FTPB);
+#define L1(S)                                                                                 \
+    case S:                                                                                   \
+        if (kind == 0) hipLaunchKernelGGL((k_fast_pass<S, 0>), grid, block, 0, st, pa);       \
+        else if (kind == 1) hipLaunchKernelGGL((k_fast_pass<S, 1>), grid, block, 0, st, pa);  \
+        else hipLaunchKernelGGL((k_fast_pass<S, 2>), grid, block, 0, st, pa);                 \
+        break;
+    switch (logS) {
+        L1(1) L1(2) L1(3) L1(4) L1(5) L1(6) L1(7) L1(8) L1(9) L1(10)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef L1
+    return hipGetLastError();
+}
+
+hipError_t launch_small_twiddles(const TwPair *tw, TwPair *small, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_small_twiddles, dim3(FT_SIZE / 2 / 256, NTT_NPRIMES), dim3(256), 0, st, tw, small);
+    return hipGetLastError();
+}

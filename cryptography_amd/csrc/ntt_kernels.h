@@ -27,22 +27,24 @@ struct MulDesc {
 /* A size class: nprod products sharing one transform length L = 2^logL.
  * Residues of product i occupy [i*L, (i+1)*L) of each (op, prime) row. */
 struct PassArgs {
-    uint64_t *res;              /* res[op][prime][nprod * L] */
+    uint64_t *res;              /* res[op][prime][len] */
     const TwPair *tw;           /* twiddle tables, 3 primes x 2 x 2^15 */
+    const TwPair *small_tw;     /* w_4096^e, e < 2048, per prime */
     const MulDesc *desc;        /* nprod descriptors (device), or nullptr: use one */
     MulDesc one;
-    size_t len;                 /* nprod * L: row length */
+    size_t len;                 /* row length: nprod * L rounded up to a tile */
+    uint32_t nprod;
     uint32_t logL;
-    uint32_t logS, logT, logC, logG;
-    int first;
+    uint32_t logT, logC, logG;  /* this pass: column stride, tile columns, tile blocks */
+    int first;                  /* read the operands' 16-bit coefficients */
+    int square;                 /* b == a: one forward transform */
+    uint64_t scale[3], scale_p[3];   /* 2^64 / L mod p and Shoup partners (middle pass) */
 };
 
-struct PointwiseArgs {
-    uint64_t *res;
-    size_t len;
-    uint64_t scale[3], scale_p[3];   /* 2^64 / L mod p and Shoup partners */
-    int square;
-};
+#define FT_TILE 4096            /* residues per transform tile */
+#define FT_MAX_LOGS 10          /* largest sub-transform of one pass */
+hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntiles, int nops, hipStream_t st);
+hipError_t launch_small_twiddles(const TwPair *tw, TwPair *small, hipStream_t st);
 
 struct GarnerConsts {
     uint64_t inv01, inv01p;      /* p0^-1 mod p1 */
@@ -66,9 +68,6 @@ struct GarnerArgs {
 };
 
 __global__ void k_twiddle_gen(TwPair *tw, uint64_t r2_0, uint64_t r2_1, uint64_t r2_2);
-__global__ void k_fwd_pass(PassArgs pa);
-__global__ void k_inv_pass(PassArgs pa);
-__global__ void k_pointwise(PointwiseArgs pw);
 __global__ void k_garner_pack(GarnerArgs ga);
 __global__ void k_carry_scan(uint8_t *blk, uint32_t nblk);
 __global__ void k_carry_apply(GarnerArgs ga);

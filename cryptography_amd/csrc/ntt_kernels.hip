@@ -1,23 +1,16 @@
 /*
  * ntt_kernels.hip — gfx950 kernels of the 3-prime CRT-NTT big-integer multiply.
  *
- * Data layout (HBM): for one product of transform length L = 2^logL the
- * residues live in one u64 array res[op][prime][L] (op 0 = a, op 1 = b).
- * Transforms are in place.  The forward transform is a multi-pass
- * decimation-in-frequency (DIF) four-step: pass i views the current block of
- * N = S*T elements as an S x T matrix (element (m, r) at r + m*T), runs the
- * S-point DIF sub-transform down every column in LDS, multiplies element
- * (m, r) by the cross twiddle w_N^(r*bitrev(m)), and leaves T-point blocks for
- * the next pass.  Spectra end up in a scrambled order that the inverse (the
- * mirror-image decimation-in-time passes) consumes directly, so no transpose
- * or bit-reversal pass exists anywhere.  Values between kernels are lazily
- * reduced to [0, 2p).
+ * Twiddle-table generation, Garner CRT with the 16-bit repack and the
+ * carry-lookahead carry, and the small-operand schoolbook.  The transform
+ * passes are in ntt_fast.hip.
  *
- * Reference counterparts: ntt/lib/ntt_kernel.hip:625-675 (stockham_kernel,
- * LDS sub-NTT), :880-918 (xtranspose_fwd/inv, cross twiddle), :386-395
- * (hadamard_kernel), ntt/lib/garner.hip:187-260 (garner_kernel),
- * ntt/lib/transfer_kernels.hip:55-313 (gather / carry-lookahead kernels) and
- * ntt/lib/arith/ntt_mul.hip:117-143 (scatter_kernel).
+ * Data layout (HBM): a size class of nprod products with transform length
+ * L = 2^logL keeps residues in res[op][prime][row], row = nprod*L rounded up
+ * to a tile (op 0 = a, op 1 = b), product i at [i*L, (i+1)*L).
+ *
+ * Reference counterparts: ntt/lib/garner.hip:187-260 (garner_kernel),
+ * ntt/lib/transfer_kernels.hip:55-313 (gather / carry-lookahead kernels).
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -26,8 +19,6 @@
 #include "ntt_kernels.h"
 
 using namespace nttc;
-
-#define TPB 256  /* threads per block for the transform passes */
 
 /* ------------------------------------------------------------------------ */
 /* Twiddle tables: tw[prime][0][j] = w^j, tw[prime][1][j] = w^(j*2^15) for a  */
@@ -88,209 +79,6 @@ __global__ void __launch_bounds__(256) k_twiddle_gen(TwPair *tw, uint64_t r2_0, 
     case 0: twiddle_gen_one<0>(tw, j, r2_0); break;
     case 1: twiddle_gen_one<1>(tw, j, r2_1); break;
     default: twiddle_gen_one<2>(tw, j, r2_2); break;
-    }
-}
-
-/* ------------------------------------------------------------------------ */
-/* Transform passes                                                           */
-/* ------------------------------------------------------------------------ */
-
-__device__ __forceinline__ uint32_t bitrev(uint32_t x, int bits)
-{
-    return bits ? (__brev(x) >> (32 - bits)) : 0;
-}
-
-/* w_N^e for e < N = 2^logN: exponent E = e * 2^(30-logN) of the 2^30 root. */
-template <int PI>
-__device__ __forceinline__ uint64_t cross_twiddle(uint64_t x, const TwPair *tw, uint32_t E)
-{
-    TwPair lo = tw[E & (NTT_TW_SIZE - 1)];
-    TwPair hi = tw[NTT_TW_SIZE + (E >> NTT_TW_LOG)];
-    x = shoup_lazy<PI>(x, lo.w, lo.wp);
-    return shoup_lazy<PI>(x, hi.w, hi.wp);
-}
-
-struct TileGeom {
-    uint32_t logS, logT, logC, logG;
-    __device__ __forceinline__ uint32_t tile_log() const { return logS + logC + logG; }
-};
-
-/* Global element index of local tile element q of tile t. */
-__device__ __forceinline__ size_t tile_addr(const TileGeom &g, uint32_t t, uint32_t q,
-                                            uint32_t &m, uint32_t &r)
-{
-    uint32_t c = q & ((1u << g.logC) - 1);
-    m = (q >> g.logC) & ((1u << g.logS) - 1);
-    uint32_t gg = q >> (g.logC + g.logS);
-    uint32_t colTiles = g.logT - g.logC;                 /* log2 column tiles per block */
-    size_t b0 = ((size_t)(t >> colTiles)) << g.logG;
-    uint32_t col0 = (t & ((1u << colTiles) - 1)) << g.logC;
-    r = col0 + c;
-    return ((b0 + gg) << (g.logS + g.logT)) + r + ((size_t)m << g.logT);
-}
-
-/* 16-bit coefficient `loc` of operand op (0: a, 1: b) of a product. */
-__device__ __forceinline__ uint64_t coeff16(const MulDesc &d, int op, size_t loc)
-{
-    const uint16_t *s = (const uint16_t *)(op ? d.b : d.a);
-    size_t n16 = 4 * (size_t)(op ? d.nb : d.na);
-    return loc < n16 ? (uint64_t)s[loc] : 0;
-}
-
-/* One forward DIF pass over tile t of residue array x (prime PI).  The first
- * pass reads the operands' 16-bit coefficients instead of x. */
-template <int PI>
-__device__ void fwd_pass_tile(uint64_t *x, const PassArgs &pa, int op, TileGeom g, uint32_t t, uint64_t *lds)
-{
-    const TwPair *tw = pa.tw + (size_t)PI * 2 * NTT_TW_SIZE;
-    const uint32_t tsz = 1u << g.tile_log();
-    const size_t lmask = ((size_t)1 << pa.logL) - 1;
-    for (uint32_t q = threadIdx.x; q < tsz; q += TPB) {
-        uint32_t m, r;
-        size_t a = tile_addr(g, t, q, m, r);
-        if (pa.first) {
-            /* first pass: read the operand's 16-bit pieces directly (the split) */
-            size_t prod = a >> pa.logL;
-            lds[q] = coeff16(pa.desc ? pa.desc[prod] : pa.one, op, a & lmask);
-        } else {
-            lds[q] = x[a];
-        }
-    }
-    __syncthreads();
-    const uint32_t S = 1u << g.logS, C = 1u << g.logC;
-    for (int lh = (int)g.logS - 1; lh >= 0; lh--) {
-        uint32_t h = 1u << lh;
-        for (uint32_t b = threadIdx.x; b < tsz / 2; b += TPB) {
-            uint32_t c = b & (C - 1);
-            uint32_t u = b >> g.logC;
-            uint32_t gg = u >> (g.logS - 1);
-            uint32_t v = u & (S / 2 - 1);
-            uint32_t j = v & (h - 1);
-            uint32_t m1 = ((v >> lh) << (lh + 1)) + j;
-            uint32_t i1 = (((gg << g.logS) + m1) << g.logC) + c;
-            uint32_t i2 = i1 + (h << g.logC);
-            TwPair w = tw[NTT_TW_SIZE + (j << (NTT_TW_LOG - 1 - lh))];
-            uint64_t a0 = lds[i1], a1 = lds[i2];
-            dif_bfly<PI>(a0, a1, w.w, w.wp);
-            lds[i1] = a0;
-            lds[i2] = a1;
-        }
-        __syncthreads();
-    }
-    const uint32_t logN = g.logS + g.logT;
-    for (uint32_t q = threadIdx.x; q < tsz; q += TPB) {
-        uint32_t m, r;
-        size_t a = tile_addr(g, t, q, m, r);
-        uint64_t v = lds[q];
-        if (g.logT) {
-            uint32_t E = (r * bitrev(m, g.logS)) << (NTT_ROOT_LOG - logN);
-            v = cross_twiddle<PI>(v, tw, E);
-        }
-        x[a] = v;
-    }
-}
-
-/* One inverse DIT pass (mirror of fwd_pass_tile). */
-template <int PI>
-__device__ void inv_pass_tile(uint64_t *x, const TwPair *twall, TileGeom g, uint32_t t, uint64_t *lds)
-{
-    const TwPair *tw = twall + (size_t)PI * 2 * NTT_TW_SIZE;
-    const uint32_t tsz = 1u << g.tile_log();
-    const uint32_t logN = g.logS + g.logT;
-    for (uint32_t q = threadIdx.x; q < tsz; q += TPB) {
-        uint32_t m, r;
-        size_t a = tile_addr(g, t, q, m, r);
-        uint64_t v = x[a];
-        if (g.logT) {
-            uint32_t E = (r * bitrev(m, g.logS)) << (NTT_ROOT_LOG - logN);
-            E = (0u - E) & ((1u << NTT_ROOT_LOG) - 1);
-            v = cross_twiddle<PI>(v, tw, E);
-        }
-        lds[q] = v;
-    }
-    __syncthreads();
-    const uint32_t S = 1u << g.logS, C = 1u << g.logC;
-    for (uint32_t lh = 0; lh < g.logS; lh++) {
-        uint32_t h = 1u << lh;
-        for (uint32_t b = threadIdx.x; b < tsz / 2; b += TPB) {
-            uint32_t c = b & (C - 1);
-            uint32_t u = b >> g.logC;
-            uint32_t gg = u >> (g.logS - 1);
-            uint32_t v = u & (S / 2 - 1);
-            uint32_t j = v & (h - 1);
-            uint32_t m1 = ((v >> lh) << (lh + 1)) + j;
-            uint32_t i1 = (((gg << g.logS) + m1) << g.logC) + c;
-            uint32_t i2 = i1 + (h << g.logC);
-            uint32_t e = (NTT_TW_SIZE - (j << (NTT_TW_LOG - 1 - lh))) & (NTT_TW_SIZE - 1);
-            TwPair w = tw[NTT_TW_SIZE + e];
-            uint64_t a0 = lds[i1], a1 = lds[i2];
-            dit_bfly<PI>(a0, a1, w.w, w.wp);
-            lds[i1] = a0;
-            lds[i2] = a1;
-        }
-        __syncthreads();
-    }
-    for (uint32_t q = threadIdx.x; q < tsz; q += TPB) {
-        uint32_t m, r;
-        size_t a = tile_addr(g, t, q, m, r);
-        x[a] = lds[q];
-    }
-}
-
-/* grid: x = tile, y = prime, z = operand.  LDS: 2^tile_log u64. */
-__global__ void __launch_bounds__(TPB) k_fwd_pass(PassArgs pa)
-{
-    extern __shared__ uint64_t lds[];
-    const uint32_t pi = blockIdx.y, op = blockIdx.z;
-    uint64_t *x = pa.res + ((size_t)op * NTT_NPRIMES + pi) * pa.len;
-    TileGeom g{pa.logS, pa.logT, pa.logC, pa.logG};
-    switch (pi) {
-    case 0: fwd_pass_tile<0>(x, pa, op, g, blockIdx.x, lds); break;
-    case 1: fwd_pass_tile<1>(x, pa, op, g, blockIdx.x, lds); break;
-    default: fwd_pass_tile<2>(x, pa, op, g, blockIdx.x, lds); break;
-    }
-}
-
-__global__ void __launch_bounds__(TPB) k_inv_pass(PassArgs pa)
-{
-    extern __shared__ uint64_t lds[];
-    const uint32_t pi = blockIdx.y;
-    uint64_t *x = pa.res + (size_t)pi * pa.len;
-    TileGeom g{pa.logS, pa.logT, pa.logC, pa.logG};
-    switch (pi) {
-    case 0: inv_pass_tile<0>(x, pa.tw, g, blockIdx.x, lds); break;
-    case 1: inv_pass_tile<1>(x, pa.tw, g, blockIdx.x, lds); break;
-    default: inv_pass_tile<2>(x, pa.tw, g, blockIdx.x, lds); break;
-    }
-}
-
-/* ------------------------------------------------------------------------ */
-/* Pointwise product with the 1/L scale: a[i] = a[i] * b[i] / L mod p.        */
-/* ------------------------------------------------------------------------ */
-
-template <int PI>
-__device__ __forceinline__ uint64_t pw_one(uint64_t a, uint64_t b, uint64_t s, uint64_t sp)
-{
-    /* mont_mul gives a*b/2^64; s = 2^64/L mod p restores the scale */
-    return shoup_lazy<PI>(mont_mul<PI>(a, b), s, sp);
-}
-
-__global__ void __launch_bounds__(256) k_pointwise(PointwiseArgs pw)
-{
-    const uint32_t pi = blockIdx.y;
-    uint64_t *a = pw.res + (size_t)pi * pw.len;
-    const uint64_t *b = pw.square ? a : pw.res + ((size_t)NTT_NPRIMES + pi) * pw.len;
-    const uint64_t s = pw.scale[pi], sp = pw.scale_p[pi];
-    size_t stride = (size_t)gridDim.x * 256;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < pw.len; i += stride) {
-        uint64_t va = a[i], vb = b[i];
-        uint64_t r;
-        switch (pi) {
-        case 0: r = pw_one<0>(va, vb, s, sp); break;
-        case 1: r = pw_one<1>(va, vb, s, sp); break;
-        default: r = pw_one<2>(va, vb, s, sp); break;
-        }
-        a[i] = r;
     }
 }
 

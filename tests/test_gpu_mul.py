@@ -130,3 +130,13 @@ def test_too_large_is_an_error():
     with pytest.raises(E.NttError) as ei:
         E.mul_device(E.DeviceBuffer(1), E.DeviceBuffer(1), 1 << 28, E.DeviceBuffer(1), 1 << 28)
     assert ei.value.code == 2
+
+
+@pytest.mark.parametrize("na,nb", [(1 << 18, (1 << 18) + 5), (3 << 17, 1 << 16)])
+def test_three_pass_transform(na, nb):
+    """Transform lengths 2^21 and above run three passes (columns, columns, rows)."""
+    a = O.splitmix(31 + na, na)
+    b = O.splitmix(37 + nb, nb)
+    assert E.log_len(na, nb) >= 21
+    assert np.array_equal(E.mul(a, b), O.gmp_mul(a, b))
+    assert np.array_equal(E.sqr(a), O.gmp_mul(a, a))
