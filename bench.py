@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the product only (for rocprofv3), print nothing else")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one 1 GiB device-to-device copy (torch) to calibrate rocprofv3 byte counters")
     return ap.parse_args()
 
 
@@ -81,6 +83,15 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        # a plain copy kernel of known size: 1 GiB read + 1 GiB written
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 2 * (1 << 30)}))
+        return
     import cryptography_amd as E
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O

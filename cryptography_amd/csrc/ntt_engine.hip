@@ -189,20 +189,17 @@ int ws_reserve(Workspace &ws, size_t res_words, size_t scratch_bytes)
 }
 
 /* Carry geometry of a class whose largest product has max_nout limbs. */
-void carry_geom(uint32_t max_nout, uint32_t &thr, uint32_t &blk)
-{
-    thr = (max_nout + CARRY_E - 1) / CARRY_E;
-    blk = (thr + 255) / 256;
-    thr = blk * 256;
-}
+/* Carry blocks per product for a class whose largest product spans max_nout
+ * limbs, and the scratch bytes one product needs (generate masks + block cells). */
+uint32_t carry_blocks(uint32_t max_nout) { return (max_nout + CB_LIMBS - 1) / CB_LIMBS; }
+size_t carry_scratch_per_prod(uint32_t blk) { return (size_t)blk * 4 * CW_ROUNDS * 8 + blk; }
 
 /* Workspace a class needs: residues and carry scratch. */
 void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t &res_words, size_t &scratch)
 {
-    uint32_t thr, blk;
-    carry_geom(max_nout, thr, blk);
+    uint32_t blk = carry_blocks(max_nout);
     res_words = (size_t)(square ? 1 : 2) * NTT_NPRIMES * row_len(logL, nprod);
-    scratch = (size_t)nprod * (thr + blk) + 64;
+    scratch = (size_t)nprod * carry_scratch_per_prod(blk) + 64;
 }
 
 /* Enqueue one size class: nprod products, transform length 2^logL, all on
@@ -248,13 +245,13 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     for (int i = 0; i < pp.npass - 1; i++) {
         geom(i, done);
         pa.first = i == 0;
-        HIP_TRY(launch_fast_pass(pp.logS[i], 0, pa, ntiles, nops, st));
+        HIP_TRY(launch_fast_pass(pp.logS[i], i == 0 ? 3 : 0, pa, ntiles, nops, st));
         HIP_TRY(mark("k_fast_pass<fwd>"));
         done += pp.logS[i];
     }
     geom(pp.npass - 1, done);
     pa.first = pp.npass == 1;
-    HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], 2, pa, ntiles, nops, st));
+    HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], pp.npass == 1 ? 4 : 2, pa, ntiles, nops, st));
     HIP_TRY(mark("k_fast_pass<mid>"));
     for (int i = pp.npass - 2; i >= 0; i--) {
         done -= pp.logS[i];
@@ -270,9 +267,9 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     ga.logL = logL;
     ga.desc = d_desc;
     ga.one = one;
-    carry_geom(max_nout, ga.thr_per_prod, ga.blk_per_prod);
+    ga.blk_per_prod = carry_blocks(max_nout);
     ga.gen = ws.scratch;
-    ga.blk = ws.scratch + (size_t)nprod * ga.thr_per_prod;
+    ga.blk = ws.scratch + (size_t)nprod * ga.blk_per_prod * 4 * CW_ROUNDS * 8;
     ga.flag = g.flag;
     ga.gc = g.gc;
     hipLaunchKernelGGL(k_garner_pack, dim3(ga.blk_per_prod, nprod), dim3(256), 0, st, ga);

@@ -46,6 +46,21 @@ __global__ void __launch_bounds__(256) k_small_twiddles(const TwPair *tw, TwPair
     small[pi * (FT_SIZE / 2) + e] = tw[(size_t)pi * 2 * NTT_TW_SIZE + NTT_TW_SIZE + (e << 3)];
 }
 
+/* LDS layout: one pad word after every 8 residues, so the strided group
+ * accesses of every radix round (stride 1, 8 or 64 residues across lanes)
+ * fall on distinct banks. */
+#define LP(i) ((i) + ((i) >> 3))
+#define FT_LDS (FT_SIZE + FT_SIZE / 8)
+
+/* XCD-aware tile order: consecutive blocks go to different XCDs (round robin
+ * over 8), so tile numbers are dealt out in 8 contiguous ranges, one per XCD;
+ * neighbouring column tiles then share their 128-byte lines in one L2. */
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
+{
+    uint32_t x = b & 7, i = b >> 3, q = n >> 3, rem = n & 7;
+    return x * q + (x < rem ? x : rem) + i;
+}
+
 __device__ __forceinline__ uint32_t brev_n(uint32_t x, int bits)
 {
     return bits ? (__brev(x) >> (32 - bits)) : 0;
@@ -100,7 +115,7 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
         uint32_t base = (((gg << LOGS) + m0) << logC) + c;
         uint64_t x[R];
 #pragma unroll
-        for (int i = 0; i < R; i++) x[i] = lds[base + ((uint32_t)(i << LD) << logC)];
+        for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
 #pragma unroll
         for (int s = 0; s < RL; s++) {
             const int half = R >> (s + 1);          /* pair distance in group index */
@@ -109,12 +124,12 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
             for (int i = 0; i < R; i++) {
                 if (i & half) continue;
                 uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);   /* (m mod hs) */
-                TwPair w = twf[e << (LOGS - 1 - lhs)];
+                TwPair w = twf[(1u << lhs) - 1 + e];
                 dif_bfly<PI>(x[i], x[i + half], w.w, w.wp);
             }
         }
 #pragma unroll
-        for (int i = 0; i < R; i++) lds[base + ((uint32_t)(i << LD) << logC)] = x[i];
+        for (int i = 0; i < R; i++) lds[LP(base + ((uint32_t)(i << LD) << logC))] = x[i];
     }
 }
 
@@ -136,7 +151,7 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
         uint32_t base = (((gg << LOGS) + m0) << logC) + c;
         uint64_t x[R];
 #pragma unroll
-        for (int i = 0; i < R; i++) x[i] = lds[base + ((uint32_t)(i << LD) << logC)];
+        for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
 #pragma unroll
         for (int s = RL - 1; s >= 0; s--) {
             const int half = R >> (s + 1);
@@ -145,12 +160,12 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
             for (int i = 0; i < R; i++) {
                 if (i & half) continue;
                 uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
-                TwPair w = twi[e << (LOGS - 1 - lhs)];
+                TwPair w = twi[(1u << lhs) - 1 + e];
                 dit_bfly<PI>(x[i], x[i + half], w.w, w.wp);
             }
         }
 #pragma unroll
-        for (int i = 0; i < R; i++) lds[base + ((uint32_t)(i << LD) << logC)] = x[i];
+        for (int i = 0; i < R; i++) lds[LP(base + ((uint32_t)(i << LD) << logC))] = x[i];
     }
 }
 
@@ -179,24 +194,29 @@ __device__ __forceinline__ void dit_all(uint64_t *lds, const TwPair *twi, uint32
     }
 }
 
-/* Stage twiddles of an S-point transform into LDS: twf[e] = w_S^e,
- * twi[e] = w_S^-e, e < S/2 (S <= 1024), or point at the global table. */
+/* Stage twiddles of an S-point transform into LDS, one contiguous run per
+ * stage so a round's lanes read consecutive entries: the stage with
+ * half-size hs = 2^l keeps w_(2hs)^e, e < hs, at [hs - 1, 2hs - 1) of twf and
+ * w_(2hs)^-e at the same place in twi (S - 1 entries each). */
 template <int PI, int LOGS>
 __device__ __forceinline__ void stage_twiddles(const TwPair *small_all, TwPair *twf, TwPair *twi)
 {
     const TwPair *sm = small_all + PI * (FT_SIZE / 2);
-    constexpr uint32_t half = 1u << (LOGS - 1);
-    constexpr int ST = FT_LOG - LOGS;      /* w_S^e = w_4096^(e << ST) */
-    for (uint32_t e = threadIdx.x; e < half; e += FTPB) {
-        twf[e] = sm[e << ST];
-        /* w_S^-e = w_S^(S/2 - e) * w_S^(S/2) = -w_S^(S/2 - e) */
+    constexpr uint32_t n = (1u << LOGS) - 1;
+    for (uint32_t idx = threadIdx.x; idx < n; idx += FTPB) {
+        uint32_t l = 31 - __clz(idx + 1);            /* stage: hs = 2^l */
+        uint32_t e = idx + 1 - (1u << l);            /* e < hs */
+        /* w_(2hs)^e = w_4096^(e * 4096 / (2hs)) */
+        uint32_t sh = FT_LOG - 1 - l;
+        twf[idx] = sm[e << sh];
         if (e == 0) {
-            twi[0] = sm[0];
+            twi[idx] = sm[0];
         } else {
-            TwPair t = sm[(half - e) << ST];
-            /* partner of p - w is floor(2^64 - w 2^64/p) = 2^64 - 1 - wp (w != 0) */
-            twi[e].w = Prime<PI>::P - t.w;
-            twi[e].wp = ~t.wp;
+            /* w^-e = -w^(hs - e) for a root of order 2hs; the Shoup partner of
+             * p - w is 2^64 - 1 - partner(w) (w != 0) */
+            TwPair t = sm[((1u << l) - e) << sh];
+            twi[idx].w = Prime<PI>::P - t.w;
+            twi[idx].wp = ~t.wp;
         }
     }
 }
@@ -215,16 +235,16 @@ __device__ __forceinline__ uint64_t first_in(const PassArgs &pa, int op, size_t 
     return loc < n16 ? (uint64_t)s[loc] : 0;
 }
 
-template <int PI, int LOGS>
+template <int PI, int LOGS, bool FIRST>
 __device__ void fwd_tile(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds, TwPair *twf)
 {
     FGeom g{pa.logT, pa.logC, pa.logG};
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
 #pragma unroll
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB, m, r;
         size_t a = ft_addr(LOGS, g, t, q, m, r);
-        lds[q] = pa.first ? first_in(pa, op, a) : x[a];
+        lds[LP(q)] = FIRST ? first_in(pa, op, a) : x[a];
     }
     __syncthreads();
     dif_all<PI, LOGS, LOGS - 1>(lds, twf, g.logC);
@@ -234,7 +254,7 @@ __device__ void fwd_tile(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds,
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB, m, r;
         size_t a = ft_addr(LOGS, g, t, q, m, r);
-        uint64_t v = lds[q];
+        uint64_t v = lds[LP(q)];
         if (g.logT) v = xtw<PI>(v, tw, (r * brev_n(m, LOGS)) << (NTT_ROOT_LOG - logN));
         x[a] = v;
     }
@@ -244,7 +264,7 @@ template <int PI, int LOGS>
 __device__ void inv_tile(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair *twi)
 {
     FGeom g{pa.logT, pa.logC, pa.logG};
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t logN = LOGS + g.logT;
     const TwPair *tw = pa.tw + (size_t)PI * 2 * NTT_TW_SIZE;
 #pragma unroll
@@ -256,7 +276,7 @@ __device__ void inv_tile(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair 
             uint32_t E = (r * brev_n(m, LOGS)) << (NTT_ROOT_LOG - logN);
             v = xtw<PI>(v, tw, (0u - E) & ((1u << NTT_ROOT_LOG) - 1));
         }
-        lds[q] = v;
+        lds[LP(q)] = v;
     }
     __syncthreads();
     dit_all<PI, LOGS, LOGS - 1>(lds, twi, g.logC);
@@ -264,36 +284,36 @@ __device__ void inv_tile(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair 
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB, m, r;
         size_t a = ft_addr(LOGS, g, t, q, m, r);
-        x[a] = lds[q];
+        x[a] = lds[LP(q)];
     }
 }
 
 /* Last forward pass of a and b, pointwise product (x 2^64/L via Montgomery and
  * a Shoup scale), first inverse pass; result into a's row.  T = 1 here. */
-template <int PI, int LOGS>
+template <int PI, int LOGS, bool FIRST>
 __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, uint64_t *lds, TwPair *twf,
                          TwPair *twi)
 {
     FGeom g{0, 0, pa.logG};
-    const uint32_t t = blockIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     uint64_t ra[FT_PER];
 #pragma unroll
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB, m, r;
         size_t a = ft_addr(LOGS, g, t, q, m, r);
-        lds[q] = pa.first ? first_in(pa, 0, a) : xa[a];
+        lds[LP(q)] = FIRST ? first_in(pa, 0, a) : xa[a];
     }
     __syncthreads();
     dif_all<PI, LOGS, LOGS - 1>(lds, twf, 0);
 #pragma unroll
-    for (int k = 0; k < FT_PER; k++) ra[k] = lds[threadIdx.x + k * FTPB];
+    for (int k = 0; k < FT_PER; k++) ra[k] = lds[LP(threadIdx.x + k * FTPB)];
     if (!pa.square) {
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < FT_PER; k++) {
             uint32_t q = threadIdx.x + k * FTPB, m, r;
             size_t a = ft_addr(LOGS, g, t, q, m, r);
-            lds[q] = pa.first ? first_in(pa, 1, a) : xb[a];
+            lds[LP(q)] = FIRST ? first_in(pa, 1, a) : xb[a];
         }
         __syncthreads();
         dif_all<PI, LOGS, LOGS - 1>(lds, twf, 0);
@@ -302,8 +322,8 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 #pragma unroll
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB;
-        uint64_t b = pa.square ? ra[k] : lds[q];
-        lds[q] = shoup_lazy<PI>(mont_mul<PI>(ra[k], b), s, sp);
+        uint64_t b = pa.square ? ra[k] : lds[LP(q)];
+        lds[LP(q)] = shoup_lazy<PI>(mont_mul<PI>(ra[k], b), s, sp);
     }
     __syncthreads();
     dit_all<PI, LOGS, LOGS - 1>(lds, twi, 0);
@@ -311,55 +331,56 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB, m, r;
         size_t a = ft_addr(LOGS, g, t, q, m, r);
-        xa[a] = lds[q];
+        xa[a] = lds[LP(q)];
     }
 }
 
-#define TW_LDS_MAX 512   /* stage-twiddle entries per direction held in LDS */
+#define TW_LDS_MAX 1024  /* stage-twiddle entries per direction held in LDS (S - 1) */
 
-/* kind 0: forward pass, 1: inverse pass, 2: fused middle pass. */
+/* kind 0: forward pass, 1: inverse pass, 2: fused middle pass,
+ * 3: first forward pass (reads the operands), 4: single fused pass. */
+template <int PI, int LOGS, int KIND>
+__device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, int op,
+                                          uint64_t *lds, TwPair *twf, TwPair *twi)
+{
+    stage_twiddles<PI, LOGS>(pa.small_tw, twf, twi);
+    __syncthreads();
+    if constexpr (KIND == 0) fwd_tile<PI, LOGS, false>(pa, xa, op, lds, twf);
+    else if constexpr (KIND == 3) fwd_tile<PI, LOGS, true>(pa, xa, op, lds, twf);
+    else if constexpr (KIND == 1) inv_tile<PI, LOGS>(pa, xa, lds, twi);
+    else if constexpr (KIND == 2) mid_tile<PI, LOGS, false>(pa, xa, xb, lds, twf, twi);
+    else mid_tile<PI, LOGS, true>(pa, xa, xb, lds, twf, twi);
+}
+
 template <int LOGS, int KIND>
 __global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
 {
-    __shared__ uint64_t lds[FT_SIZE];
-    __shared__ TwPair twf[TW_LDS_MAX], twi[TW_LDS_MAX];
-    static_assert((1 << (LOGS - 1)) <= TW_LDS_MAX, "stage twiddles exceed LDS table");
+    __shared__ uint64_t lds[FT_LDS];
+    __shared__ TwPair twf[1 << LOGS], twi[1 << LOGS];
+    static_assert((1 << LOGS) <= TW_LDS_MAX, "stage twiddles exceed LDS table");
     const uint32_t pi = blockIdx.y, op = blockIdx.z;
     uint64_t *xa = pa.res + ((size_t)op * NTT_NPRIMES + pi) * pa.len;
     const uint64_t *xb = pa.res + ((size_t)NTT_NPRIMES + pi) * pa.len;
-    switch (pi) {
-#define CASE_PI(P)                                                                     \
-    case P:                                                                            \
-        stage_twiddles<P, LOGS>(pa.small_tw, twf, twi);                                \
-        __syncthreads();                                                               \
-        if (KIND == 0) fwd_tile<P, LOGS>(pa, xa, op, lds, twf);                        \
-        else if (KIND == 1) inv_tile<P, LOGS>(pa, xa, lds, twi);                       \
-        else mid_tile<P, LOGS>(pa, xa, xb, lds, twf, twi);                             \
-        break;
-        CASE_PI(0)
-        CASE_PI(1)
-    default:
-        stage_twiddles<2, LOGS>(pa.small_tw, twf, twi);
-        __syncthreads();
-        if (KIND == 0) fwd_tile<2, LOGS>(pa, xa, op, lds, twf);
-        else if (KIND == 1) inv_tile<2, LOGS>(pa, xa, lds, twi);
-        else mid_tile<2, LOGS>(pa, xa, xb, lds, twf, twi);
-        break;
-#undef CASE_PI
-    }
+    if (pi == 0) pass_body<0, LOGS, KIND>(pa, xa, xb, op, lds, twf, twi);
+    else if (pi == 1) pass_body<1, LOGS, KIND>(pa, xa, xb, op, lds, twf, twi);
+    else pass_body<2, LOGS, KIND>(pa, xa, xb, op, lds, twf, twi);
 }
 
 /* Host-side launcher: one pass of a class (grid.x tiles, grid.y primes,
  * grid.z operands for forward passes). */
 hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntiles, int nops, hipStream_t st)
 {
-    dim3 grid((unsigned)ntiles, NTT_NPRIMES, kind == 0 ? nops : 1);
+    dim3 grid((unsigned)ntiles, NTT_NPRIMES, (kind == 0 || kind == 3) ? nops : 1);
     dim3 block(FTPB);
-#define L1(S)                                                                                 \
-    case S:                                                                                   \
-        if (kind == 0) hipLaunchKernelGGL((k_fast_pass<S, 0>), grid, block, 0, st, pa);       \
-        else if (kind == 1) hipLaunchKernelGGL((k_fast_pass<S, 1>), grid, block, 0, st, pa);  \
-        else hipLaunchKernelGGL((k_fast_pass<S, 2>), grid, block, 0, st, pa);                 \
+#define L1(S)                                                                                  \
+    case S:                                                                                    \
+        switch (kind) {                                                                        \
+        case 0: hipLaunchKernelGGL((k_fast_pass<S, 0>), grid, block, 0, st, pa); break;        \
+        case 1: hipLaunchKernelGGL((k_fast_pass<S, 1>), grid, block, 0, st, pa); break;        \
+        case 2: hipLaunchKernelGGL((k_fast_pass<S, 2>), grid, block, 0, st, pa); break;        \
+        case 3: hipLaunchKernelGGL((k_fast_pass<S, 3>), grid, block, 0, st, pa); break;        \
+        default: hipLaunchKernelGGL((k_fast_pass<S, 4>), grid, block, 0, st, pa); break;       \
+        }                                                                                      \
         break;
     switch (logS) {
         L1(1) L1(2) L1(3) L1(4) L1(5) L1(6) L1(7) L1(8) L1(9) L1(10)

@@ -14,7 +14,9 @@ struct __attribute__((aligned(16))) TwPair {
     uint64_t w, wp;
 };
 
-#define CARRY_E 8   /* output limbs per thread in the carry kernels */
+#define CW_ROUNDS 4                     /* carry kernels: rounds of 64 limbs per wavefront */
+#define CW_LIMBS (64 * CW_ROUNDS)       /* limbs per wavefront */
+#define CB_LIMBS (4 * CW_LIMBS)         /* limbs per 256-thread block */
 
 /* One product of a batch: c[0..n_out) = a*b (+ add).  Device pointers. */
 struct MulDesc {
@@ -59,9 +61,8 @@ struct GarnerArgs {
     uint32_t logL;
     const MulDesc *desc;        /* or nullptr: use one */
     MulDesc one;
-    uint32_t thr_per_prod;      /* carry threads reserved per product */
     uint32_t blk_per_prod;      /* carry blocks per product */
-    uint8_t *gen;               /* one byte of generate bits per thread */
+    uint8_t *gen;               /* u64 generate mask per 64-limb round (8-byte aligned) */
     uint8_t *blk;               /* block aggregates, then block carry-ins */
     int *flag;                  /* integrity flag: set on a corrupted product */
     GarnerConsts gc;

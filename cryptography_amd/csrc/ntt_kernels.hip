@@ -90,13 +90,12 @@ __global__ void __launch_bounds__(256) k_twiddle_gen(TwPair *tw, uint64_t r2_0, 
  * the product coefficients are < L*(2^16-1)^2 < 2^62 < p0, so a1 = a2 = 0 for
  * an intact product and the value is a0.  Any nonzero a1/a2 is a corrupted
  * product and raises the integrity flag. */
-__device__ __forceinline__ uint64_t garner_coeff(const uint64_t *r0, const uint64_t *r1, const uint64_t *r2,
-                                                 size_t k, const GarnerConsts &gc, int &bad)
+__device__ __forceinline__ uint64_t garner3(uint64_t r0, uint64_t r1, uint64_t r2, const GarnerConsts &gc, int &bad)
 {
-    const uint64_t p0 = Prime<0>::P, p1 = Prime<1>::P, p2 = Prime<2>::P;
-    uint64_t x0 = reduce1<0>(r0[k]);
-    uint64_t x1 = reduce1<1>(r1[k]);
-    uint64_t x2 = reduce1<2>(r2[k]);
+    const uint64_t p1 = Prime<1>::P, p2 = Prime<2>::P;
+    uint64_t x0 = reduce1<0>(r0);
+    uint64_t x1 = reduce1<1>(r1);
+    uint64_t x2 = reduce1<2>(r2);
     uint64_t a0 = x0;
     uint64_t a0m1 = a0 >= p1 ? a0 - p1 : a0;            /* a0 < p0 < 2*p1 */
     uint64_t t1 = x1 >= a0m1 ? x1 - a0m1 : x1 + p1 - a0m1;
@@ -106,7 +105,6 @@ __device__ __forceinline__ uint64_t garner_coeff(const uint64_t *r0, const uint6
     b2 = b2 >= p2 ? b2 - p2 : b2;
     uint64_t t2 = x2 >= b2 ? x2 - b2 : x2 + p2 - b2;
     uint64_t a2 = reduce1<2>(shoup_lazy<2>(t2, gc.inv012, gc.inv012p));
-    (void)p0;
     bad |= (a1 | a2) != 0;
     return a0;
 }
@@ -133,14 +131,70 @@ __device__ __forceinline__ void cla_block_scan(uint8_t *sg)
     }
 }
 
-/* K_a (grid: carry blocks x products): each thread owns CARRY_E consecutive
- * output limbs of one product.  s_j = sum_t X_{4j+t} 2^(16t) (+ addend_j) is
- * split into lo_j + hi_j 2^64; res_j = lo_j + hi_{j-1} mod 2^64 is written to
- * out[j], its overflow bit (generate) to gen[], and the block's (G,P)
- * aggregate to blk[]. */
+/* ---- Garner + repack + carry: wavefront layout ------------------------------
+ * A block is 4 wavefronts; wavefront w owns CW_LIMBS consecutive output limbs,
+ * taken in CW_ROUNDS rounds of 64 (lane l of round e: limb base + 64e + l), so
+ * every load and store is a contiguous 64-lane access.  Within a round the
+ * thin carries follow from two ballots: with X = G|P and Y = G (generate and
+ * propagate bits of the 64 limbs), the carry into lane i is bit i of
+ * (X + Y + cin) ^ X ^ Y and the round's carry out is the overflow bit — the
+ * generate/propagate recurrence of ntt/lib/transfer_core.h:165-252 evaluated
+ * by the adder itself. */
+
+/* (lo, hi) = sum_{t<4} X_{4j+t} 2^(16t) for limb j, with Garner per coefficient */
+__device__ __forceinline__ void limb_sum(const uint64_t *r0, const uint64_t *r1, const uint64_t *r2, size_t j,
+                                         size_t n_coef, const GarnerConsts &gc, int &bad, uint64_t &lo,
+                                         uint64_t &hi)
+{
+    lo = 0;
+    hi = 0;
+    size_t k0 = 4 * j;
+    if (k0 >= n_coef) return;
+    uint64_t v0[4], v1[4], v2[4];
+    if (k0 + 4 <= n_coef) {
+        const ulonglong2 *p0 = (const ulonglong2 *)(r0 + k0), *p1 = (const ulonglong2 *)(r1 + k0),
+                         *p2 = (const ulonglong2 *)(r2 + k0);
+        ulonglong2 a0 = p0[0], a1 = p0[1], b0 = p1[0], b1 = p1[1], c0 = p2[0], c1 = p2[1];
+        v0[0] = a0.x; v0[1] = a0.y; v0[2] = a1.x; v0[3] = a1.y;
+        v1[0] = b0.x; v1[1] = b0.y; v1[2] = b1.x; v1[3] = b1.y;
+        v2[0] = c0.x; v2[1] = c0.y; v2[2] = c1.x; v2[3] = c1.y;
+    } else {
+        for (int t = 0; t < 4; t++) {
+            bool in = k0 + t < n_coef;
+            v0[t] = in ? r0[k0 + t] : 0;
+            v1[t] = in ? r1[k0 + t] : 0;
+            v2[t] = in ? r2[k0 + t] : 0;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        uint64_t X = garner3(v0[t], v1[t], v2[t], gc, bad);
+        if (t == 0) {
+            lo = X;
+        } else {
+            uint64_t sl = X << (16 * t), sh = X >> (64 - 16 * t);
+            uint64_t nl = lo + sl;
+            hi += sh + (nl < lo);
+            lo = nl;
+        }
+    }
+}
+
+/* Round aggregate (generate, propagate) from the ballot masks. */
+__device__ __forceinline__ uint8_t round_cell(uint64_t G, uint64_t P)
+{
+    uint64_t X = G | P, Y = G;
+    uint64_t s = X + Y;
+    uint8_t gen = s < X;                 /* overflow with carry-in 0 */
+    uint8_t prop = (s == ~0ull) && !gen; /* overflow only with carry-in 1 */
+    return (uint8_t)(gen | (prop << 1));
+}
+
+/* K_a (grid: blocks x products).  Writes res_j (pre-carry digit) to c[j], the
+ * round generate masks to gen[], the block's (G,P) aggregate to blk[]. */
 __global__ void __launch_bounds__(256) k_garner_pack(GarnerArgs ga)
 {
-    __shared__ uint8_t sg[256];
+    __shared__ uint8_t wcell[4];
     const uint32_t prod = blockIdx.y;
     const MulDesc d = ga.desc ? ga.desc[prod] : ga.one;
     const size_t L = (size_t)1 << ga.logL;
@@ -148,61 +202,58 @@ __global__ void __launch_bounds__(256) k_garner_pack(GarnerArgs ga)
     const uint64_t *r0 = ga.res + off, *r1 = ga.res + ga.len + off, *r2 = ga.res + 2 * ga.len + off;
     const size_t n_out = d.n_out;
     const size_t n_coef = 4 * ((size_t)d.na + d.nb) < L ? 4 * ((size_t)d.na + d.nb) : L;
-    const size_t thr = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t j0 = thr * CARRY_E;
-    int bad = 0;
-    uint8_t G = 0, P = 1, gbits = 0;
-    /* limbs at or above n_out that the convolution reaches must be zero */
     const size_t n_span = (n_coef + 3) / 4;
-    if (j0 >= n_out && j0 < n_span) {
-        for (size_t k = 4 * j0; k < 4 * (j0 + CARRY_E) && k < n_coef; k++)
-            bad |= garner_coeff(r0, r1, r2, k, ga.gc, bad) != 0;
-    }
-    if (j0 < n_out) {
-        uint64_t prev_hi = 0;
-        for (int jj = -1; jj < CARRY_E; jj++) {
-            if (jj == -1 && j0 == 0) continue;
-            size_t j = j0 + jj;
-            if (j >= n_out) break;
-            uint64_t lo = 0, hi = 0;
-            for (int t = 0; t < 4; t++) {
-                size_t k = 4 * j + t;
-                if (k >= n_coef) break;
-                uint64_t X = garner_coeff(r0, r1, r2, k, ga.gc, bad);
-                if (t == 0) {
-                    lo = X;
-                } else {
-                    uint64_t sl = X << (16 * t), sh = X >> (64 - 16 * t);
-                    uint64_t nl = lo + sl;
-                    hi += sh + (nl < lo);
-                    lo = nl;
-                }
-            }
-            if (d.add && j < d.nadd) {
-                uint64_t nl = lo + d.add[j];
-                hi += nl < lo;
-                lo = nl;
-            }
-            if (jj >= 0) {
-                uint64_t res = lo + prev_hi;
-                uint8_t g = res < lo;
-                uint8_t p = res == ~0ull;
-                d.c[j] = res;
-                gbits |= (uint8_t)(g << jj);
-                G = g | (p & G);
-                P = P & p;
-            }
-            prev_hi = hi;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const size_t wave = (size_t)blockIdx.x * 4 + w;
+    const size_t base = wave * CW_LIMBS;
+    uint64_t *gmask = (uint64_t *)ga.gen + ((size_t)prod * ga.blk_per_prod * 4 + wave) * CW_ROUNDS;
+    int bad = 0;
+    uint64_t prev_hi = 0;   /* hi of limb base-1, valid in lane 0 */
+    if (lane == 0 && base > 0 && base - 1 < n_span) {
+        uint64_t lo;
+        limb_sum(r0, r1, r2, base - 1, n_coef, ga.gc, bad, lo, prev_hi);
+        if (d.add && base - 1 < d.nadd) {
+            uint64_t nl = lo + d.add[base - 1];
+            prev_hi += nl < lo;
         }
-        /* the high part of the top limb must vanish */
-        if (j0 + CARRY_E >= n_out) bad |= prev_hi != 0;
-        ga.gen[(size_t)prod * ga.thr_per_prod + thr] = gbits;
+    }
+    uint8_t agg = 2;    /* identity (G=0, P=1) */
+#pragma unroll 1
+    for (int e = 0; e < CW_ROUNDS; e++) {
+        size_t j = base + 64 * e + lane;
+        uint64_t lo = 0, hi = 0;
+        if (j < n_span) limb_sum(r0, r1, r2, j, n_coef, ga.gc, bad, lo, hi);
+        if (d.add && j < d.nadd) {
+            uint64_t nl = lo + d.add[j];
+            hi += nl < lo;
+            lo = nl;
+        }
+        uint64_t up = __shfl_up(hi, 1);
+        uint64_t hprev = lane ? up : prev_hi;
+        prev_hi = __shfl(hi, 63);
+        uint64_t res = lo + hprev;
+        bool g = res < lo, p = res == ~0ull;
+        if (j < n_out) {
+            d.c[j] = res;
+            if (j == n_out - 1) bad |= hi != 0;          /* nothing may spill past the top limb */
+        } else {
+            bad |= (lo | hi | (j == n_out ? hprev : 0)) != 0;
+            g = false;
+            p = true;                                       /* identity cells */
+        }
+        uint64_t G = __ballot(g), P = __ballot(p);
+        if (lane == 0) gmask[e] = G;
+        uint8_t cell = round_cell(G, P);
+        agg = cla_compose(agg, cell);
     }
     if (bad) ga.flag[0] = 1;
-    sg[threadIdx.x] = (uint8_t)(G | (P << 1));
+    if (lane == 0) wcell[w] = agg;
     __syncthreads();
-    cla_block_scan<256>(sg);
-    if (threadIdx.x == 255) ga.blk[(size_t)prod * ga.blk_per_prod + blockIdx.x] = sg[255];
+    if (threadIdx.x == 0) {
+        uint8_t b = 2;
+        for (int k = 0; k < 4; k++) b = cla_compose(b, wcell[k]);
+        ga.blk[(size_t)prod * ga.blk_per_prod + blockIdx.x] = b;
+    }
 }
 
 /* K_b (grid: products): exclusive carry-in of every block of one product. */
@@ -226,56 +277,51 @@ __global__ void __launch_bounds__(1024) k_carry_scan(uint8_t *blkall, uint32_t n
     }
 }
 
-/* K_c (grid: carry blocks x products): thread carry-in = block carry-in
- * composed with the in-block exclusive prefix; then ripple through the
- * thread's limbs. */
+/* K_c (grid: blocks x products): carry-in of each wavefront = block carry-in
+ * composed with the earlier wavefronts' aggregates; each round then adds its
+ * carry vector (X + Y + cin) ^ X ^ Y and passes the overflow on. */
 __global__ void __launch_bounds__(256) k_carry_apply(GarnerArgs ga)
 {
-    __shared__ uint8_t sg[256];
+    __shared__ uint8_t wcell[4];
     const uint32_t prod = blockIdx.y;
     const MulDesc d = ga.desc ? ga.desc[prod] : ga.one;
     const size_t n_out = d.n_out;
-    const size_t thr = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t j0 = thr * CARRY_E;
-    uint8_t G = 0, P = 1, gbits = 0;
-    uint64_t v[CARRY_E];
-    if (j0 < n_out) {
-        gbits = ga.gen[(size_t)prod * ga.thr_per_prod + thr];
-        for (int jj = 0; jj < CARRY_E; jj++) {
-            size_t j = j0 + jj;
-            v[jj] = j < n_out ? d.c[j] : 0;
-            uint8_t g = (gbits >> jj) & 1;
-            uint8_t p = j < n_out && v[jj] == ~0ull;
-            G = g | (p & G);
-            P = P & p;
-        }
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const size_t wave = (size_t)blockIdx.x * 4 + w;
+    const size_t base = wave * CW_LIMBS;
+    const uint64_t *gmask = (const uint64_t *)ga.gen + ((size_t)prod * ga.blk_per_prod * 4 + wave) * CW_ROUNDS;
+    uint64_t v[CW_ROUNDS], G[CW_ROUNDS], P[CW_ROUNDS];
+    uint8_t agg = 2;
+#pragma unroll
+    for (int e = 0; e < CW_ROUNDS; e++) {
+        size_t j = base + 64 * e + lane;
+        v[e] = j < n_out ? d.c[j] : 0;
+        G[e] = gmask[e];
+        P[e] = __ballot(j >= n_out || v[e] == ~0ull);
+        agg = cla_compose(agg, round_cell(G[e], P[e]));
     }
-    sg[threadIdx.x] = (uint8_t)(G | (P << 1));
+    if (lane == 0) wcell[w] = agg;
     __syncthreads();
-    cla_block_scan<256>(sg);
-    uint8_t bc = ga.blk[(size_t)prod * ga.blk_per_prod + blockIdx.x];
-    uint8_t cin;
-    if (threadIdx.x == 0) cin = bc;
-    else {
-        uint8_t e = sg[threadIdx.x - 1];
-        cin = (e & 1) | ((e >> 1) & bc);
+    uint8_t c = ga.blk[(size_t)prod * ga.blk_per_prod + blockIdx.x];   /* block carry-in */
+    for (uint32_t k = 0; k < w; k++) {
+        uint8_t x = wcell[k];
+        c = (x & 1) | ((x >> 1) & c);
     }
-    if (j0 < n_out) {
-        for (int jj = 0; jj < CARRY_E; jj++) {
-            size_t j = j0 + jj;
-            if (j >= n_out) {
-                if (cin) ga.flag[0] = 1;     /* carry out of the top limb */
-                cin = 0;
-                break;
-            }
-            uint64_t r = v[jj];
-            uint8_t g = (gbits >> jj) & 1;
-            uint8_t p = r == ~0ull;
-            d.c[j] = r + cin;
-            cin = g | (p & cin);
-        }
-        if (cin && j0 + CARRY_E == n_out) ga.flag[0] = 1;
+    uint64_t cin = c & 1;
+#pragma unroll
+    for (int e = 0; e < CW_ROUNDS; e++) {
+        size_t j = base + 64 * e + lane;
+        uint64_t X = G[e] | P[e], Y = G[e];
+        uint64_t s = X + Y;
+        uint64_t s2 = s + cin;
+        uint64_t carries = s2 ^ X ^ Y;
+        uint64_t cout = (s < X) | (s2 < s);
+        uint64_t cj = (carries >> lane) & 1;
+        if (j < n_out) d.c[j] = v[e] + cj;
+        else if (j == n_out && cj) ga.flag[0] = 1;        /* carry out of the top limb */
+        cin = cout;
     }
+    if (cin && base + CW_LIMBS == (size_t)n_out) ga.flag[0] = 1;
 }
 
 /* ------------------------------------------------------------------------ */
