@@ -96,6 +96,8 @@ int log_len(size_t na, size_t nb);
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);
 int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hipStream_t st);
 int sync_check();
+int copy_h2d(void *dst, const void *src, size_t bytes);
+int copy_d2h(void *dst, const void *src, size_t bytes);
 
 }  // namespace ntte
 

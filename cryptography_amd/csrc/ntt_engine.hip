@@ -68,6 +68,8 @@ uint64_t h_shoup(uint64_t w, uint64_t p) { return (uint64_t)(((u128)w << 64) / p
 int grow_dev(uint64_t **buf, size_t *have, size_t words)
 {
     if (*have >= words) return NTT_OK;
+    /* queued work may still read the old buffer */
+    if (*buf && g.stream) (void)hipStreamSynchronize(g.stream);
     if (*buf) (void)hipFree(*buf);
     *buf = nullptr;
     *have = 0;
@@ -137,6 +139,21 @@ void engine_teardown()
     g.threshold = thr;
 }
 
+/* Copies ordered after everything queued on the engine stream (the stream is
+ * non-blocking, so a plain hipMemcpy would not wait for its kernels). */
+int copy_h2d(void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return NTT_OK;
+}
+int copy_d2h(void *dst, const void *src, size_t bytes)
+{
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g.stream));
+    HIP_TRY(hipStreamSynchronize(g.stream));
+    return NTT_OK;
+}
+
 int log_len(size_t na, size_t nb)
 {
     size_t need = 4 * (na + nb);
@@ -177,6 +194,7 @@ int ws_reserve(Workspace &ws, size_t res_words, size_t scratch_bytes)
     int rc = grow_dev(&ws.res, &ws.res_words, res_words);
     if (rc) return rc;
     if (ws.scratch_bytes < scratch_bytes) {
+        if (ws.scratch && g.stream) (void)hipStreamSynchronize(g.stream);
         if (ws.scratch) (void)hipFree(ws.scratch);
         ws.scratch = nullptr;
         ws.scratch_bytes = 0;
@@ -274,7 +292,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     ga.gc = g.gc;
     hipLaunchKernelGGL(k_garner_pack, dim3(ga.blk_per_prod, nprod), dim3(256), 0, st, ga);
     HIP_TRY(mark("k_garner_pack"));
-    hipLaunchKernelGGL(k_carry_scan, dim3(nprod), dim3(1024), 0, st, ga.blk, ga.blk_per_prod);
+    hipLaunchKernelGGL(k_carry_scan, dim3(nprod), dim3(1024), 0, st, ga.blk, ga.blk_per_prod, (uint8_t)0);
     HIP_TRY(mark("k_carry_scan"));
     hipLaunchKernelGGL(k_carry_apply, dim3(ga.blk_per_prod, nprod), dim3(256), 0, st, ga);
     HIP_TRY(mark("k_carry_apply"));
@@ -432,7 +450,8 @@ int sync_check()
     }
     g.timing_pending = 0;
     int flag = 0;
-    HIP_TRY(hipMemcpy(&flag, g.flag, sizeof(int), hipMemcpyDeviceToHost));
+    int rcf = copy_d2h(&flag, g.flag, sizeof(int));
+    if (rcf) return rcf;
     if (flag) {
         HIP_TRY(hipMemset(g.flag, 0, sizeof(int)));
         return fail(NTT_ERR_INTEGRITY, "CRT integrity check failed: a product coefficient exceeded its bound");
@@ -539,13 +558,14 @@ int ntt_mul_batch_device_async(const ntt_mul_desc *descs, size_t n)
     /* the descriptor buffer may still be read by an earlier batch */
     HIP_TRY(hipStreamSynchronize(g.stream));
     if (g.d_desc_n < n) {
-        if (g.d_desc) (void)hipFree(g.d_desc);
+        if (g.d_desc) (void)hipFree(g.d_desc);   /* the stream was synchronized above */
         g.d_desc = nullptr;
         g.d_desc_n = 0;
         HIP_TRY(hipMalloc((void **)&g.d_desc, n * sizeof(MulDesc)));
         g.d_desc_n = n;
     }
-    HIP_TRY(hipMemcpy(g.d_desc, bp.sorted.data(), n * sizeof(MulDesc), hipMemcpyHostToDevice));
+    rc = copy_h2d(g.d_desc, bp.sorted.data(), n * sizeof(MulDesc));
+    if (rc) return rc;
     rc = ws_reserve(g_ws, bp.res_words, bp.scratch);
     if (rc) return rc;
     g.dispatches += n;
@@ -597,8 +617,7 @@ int ntt_batch_graph_create(const ntt_mul_desc *descs, size_t n, ntt_batch_graph 
     if (!rc) rc = ws_reserve(gr->ws, gr->bp.res_words ? gr->bp.res_words : 1, gr->bp.scratch ? gr->bp.scratch : 64);
     if (!rc && hipMalloc((void **)&gr->d_desc, n * sizeof(MulDesc)) != hipSuccess)
         rc = fail(NTT_ERR_NOMEM, "descriptor allocation failed");
-    if (!rc && hipMemcpy(gr->d_desc, gr->bp.sorted.data(), n * sizeof(MulDesc), hipMemcpyHostToDevice) != hipSuccess)
-        rc = fail(NTT_ERR_HIP, "descriptor upload failed");
+    if (!rc) rc = copy_h2d(gr->d_desc, gr->bp.sorted.data(), n * sizeof(MulDesc));
     if (!rc && hipStreamSynchronize(g.stream) != hipSuccess) rc = fail(NTT_ERR_HIP, "stream sync failed");
     if (rc) {
         ntt_batch_graph_destroy(gr);
@@ -668,13 +687,15 @@ void ntt_device_free(void *p)
 }
 int ntt_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
-    return NTT_OK;
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    return copy_h2d(dst, src, bytes);
 }
 int ntt_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
-    HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-    return NTT_OK;
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    return copy_d2h(dst, src, bytes);
 }
 
 /* bigint_alloc_managed storage (cf. ntt/lib/arith/bigint_hip_alloc.hip). */
@@ -702,8 +723,7 @@ int ntt_engine_read_twiddles(uint64_t *out, size_t n_words)
     if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
     size_t need = (size_t)NTT_NPRIMES * 2 * NTT_TW_SIZE * 2;
     if (n_words < need) return fail(NTT_ERR_ARG, "twiddle buffer too small (%zu < %zu)", n_words, need);
-    HIP_TRY(hipMemcpy(out, g.tw, need * 8, hipMemcpyDeviceToHost));
-    return NTT_OK;
+    return copy_d2h(out, g.tw, need * 8);
 }
 
 }  // extern "C"

@@ -70,7 +70,23 @@ struct GarnerArgs {
 
 __global__ void k_twiddle_gen(TwPair *tw, uint64_t r2_0, uint64_t r2_1, uint64_t r2_2);
 __global__ void k_garner_pack(GarnerArgs ga);
-__global__ void k_carry_scan(uint8_t *blk, uint32_t nblk);
+/* Block carry-ins from block (G,P) cells; cin0 is the carry into block 0. */
+__global__ void k_carry_scan(uint8_t *blk, uint32_t nblk, uint8_t cin0);
+__device__ __forceinline__ uint8_t cla_compose(uint8_t lo, uint8_t hi)
+{
+    uint8_t g = (hi & 1) | ((hi >> 1) & lo & 1);
+    uint8_t p = (uint8_t)((hi >> 1) & (lo >> 1) & 1);
+    return (uint8_t)(g | (p << 1));
+}
+/* Round cell (generate, propagate) of 64 limbs from their ballot masks. */
+__device__ __forceinline__ uint8_t round_cell(uint64_t G, uint64_t P)
+{
+    uint64_t X = G | P, Y = G;
+    uint64_t s = X + Y;
+    uint8_t gen = s < X;
+    uint8_t prop = (s == ~0ull) && !gen;
+    return (uint8_t)(gen | (prop << 1));
+}
 __global__ void k_carry_apply(GarnerArgs ga);
 /* Batched schoolbook: one wavefront per product, 4 products per block. */
 #define SB_MAX_OUT 256

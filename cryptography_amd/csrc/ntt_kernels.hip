@@ -109,15 +109,6 @@ __device__ __forceinline__ uint64_t garner3(uint64_t r0, uint64_t r1, uint64_t r
     return a0;
 }
 
-/* (G,P) cell composition: lower-index cell `lo` then higher `hi`;
- * identity is (0,1), encoded G | P<<1 = 2. */
-__device__ __forceinline__ uint8_t cla_compose(uint8_t lo, uint8_t hi)
-{
-    uint8_t g = (hi & 1) | ((hi >> 1) & lo & 1);
-    uint8_t p = (uint8_t)((hi >> 1) & (lo >> 1) & 1);
-    return (uint8_t)(g | (p << 1));
-}
-
 /* Inclusive scan of 256 (or 1024) cells in LDS, lower thread index first. */
 template <int N>
 __device__ __forceinline__ void cla_block_scan(uint8_t *sg)
@@ -178,16 +169,6 @@ __device__ __forceinline__ void limb_sum(const uint64_t *r0, const uint64_t *r1,
             lo = nl;
         }
     }
-}
-
-/* Round aggregate (generate, propagate) from the ballot masks. */
-__device__ __forceinline__ uint8_t round_cell(uint64_t G, uint64_t P)
-{
-    uint64_t X = G | P, Y = G;
-    uint64_t s = X + Y;
-    uint8_t gen = s < X;                 /* overflow with carry-in 0 */
-    uint8_t prop = (s == ~0ull) && !gen; /* overflow only with carry-in 1 */
-    return (uint8_t)(gen | (prop << 1));
 }
 
 /* K_a (grid: blocks x products).  Writes res_j (pre-carry digit) to c[j], the
@@ -257,7 +238,7 @@ __global__ void __launch_bounds__(256) k_garner_pack(GarnerArgs ga)
 }
 
 /* K_b (grid: products): exclusive carry-in of every block of one product. */
-__global__ void __launch_bounds__(1024) k_carry_scan(uint8_t *blkall, uint32_t nblk)
+__global__ void __launch_bounds__(1024) k_carry_scan(uint8_t *blkall, uint32_t nblk, uint8_t cin0)
 {
     __shared__ uint8_t sg[1024];
     uint8_t *blk = blkall + (size_t)blockIdx.x * nblk;
@@ -268,7 +249,8 @@ __global__ void __launch_bounds__(1024) k_carry_scan(uint8_t *blkall, uint32_t n
     sg[threadIdx.x] = agg;
     __syncthreads();
     cla_block_scan<1024>(sg);
-    uint8_t cin = threadIdx.x ? (sg[threadIdx.x - 1] & 1) : 0;
+    /* carry into this thread's segment: the prefix before it, fed by cin0 */
+    uint8_t cin = threadIdx.x ? (uint8_t)((sg[threadIdx.x - 1] & 1) | ((sg[threadIdx.x - 1] >> 1) & cin0)) : cin0;
     __syncthreads();
     for (uint32_t i = s0; i < s0 + per && i < nblk; i++) {
         uint8_t c = blk[i];

@@ -208,9 +208,7 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
     t->n_leaves = (uint32_t)leaves.size();
     if (!rc && hipMalloc((void **)&t->d_leaves, leaves.size() * sizeof(LeafDesc)) != hipSuccess)
         rc = fail(NTT_ERR_NOMEM, "leaf descriptors");
-    if (!rc && hipMemcpy(t->d_leaves, leaves.data(), leaves.size() * sizeof(LeafDesc), hipMemcpyHostToDevice) !=
-                   hipSuccess)
-        rc = fail(NTT_ERR_HIP, "leaf descriptor upload");
+    if (!rc) rc = copy_h2d(t->d_leaves, leaves.data(), leaves.size() * sizeof(LeafDesc));
     /* one batch per height: B = B_L*B_R and P = P_L*B_R + P_R */
     std::vector<std::vector<ntt_mul_desc>> per(H + 1);
     for (auto &nd : t->nodes) {
@@ -237,10 +235,9 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
         if (hipMalloc((void **)&t->d_desc, ndesc * sizeof(MulDesc)) != hipSuccess)
             rc = fail(NTT_ERR_NOMEM, "tree descriptors");
         for (int h = 1; h <= H && !rc; h++)
-            if (!t->levels[h].sorted.empty() &&
-                hipMemcpy(t->d_desc + t->level_off[h], t->levels[h].sorted.data(),
-                          t->levels[h].sorted.size() * sizeof(MulDesc), hipMemcpyHostToDevice) != hipSuccess)
-                rc = fail(NTT_ERR_HIP, "tree descriptor upload");
+            if (!t->levels[h].sorted.empty())
+                rc = copy_h2d(t->d_desc + t->level_off[h], t->levels[h].sorted.data(),
+                              t->levels[h].sorted.size() * sizeof(MulDesc));
     }
     if (!rc) rc = ws_reserve(t->ws, res_words, scratch);
     if (!rc && use_graph) {
@@ -314,8 +311,9 @@ int ntt_split_tree_result(ntt_split_tree *t, uint64_t *P, size_t capP, size_t *n
     const TreeNode &r = t->nodes[0];
     if (capP < r.capP || capB < r.capB)
         return fail(NTT_ERR_ARG, "result buffers too small (need %u and %u limbs)", r.capP, r.capB);
-    HIP_TRY(hipMemcpy(P, t->arena + r.offP, (size_t)r.capP * 8, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(B, t->arena + r.offB, (size_t)r.capB * 8, hipMemcpyDeviceToHost));
+    int rc = copy_d2h(P, t->arena + r.offP, (size_t)r.capP * 8);
+    if (!rc) rc = copy_d2h(B, t->arena + r.offB, (size_t)r.capB * 8);
+    if (rc) return rc;
     size_t np = r.capP, nb = r.capB;
     while (np && P[np - 1] == 0) np--;
     while (nb && B[nb - 1] == 0) nb--;

@@ -280,3 +280,21 @@ def binary_split_ref_api(a, b, batched=False):
     P, B = _bigint_to_np(sr.P), _bigint_to_np(sr.B)
     L.split_free(ctypes.byref(sr))
     return P, B
+
+
+class EStats(ctypes.Structure):
+    _fields_ = [("t_tree", ctypes.c_double), ("t_recip", ctypes.c_double), ("t_div", ctypes.c_double),
+                ("t_radix", ctypes.c_double), ("t_total", ctypes.c_double), ("n_terms", ctypes.c_long),
+                ("products", ctypes.c_ulong)]
+
+
+def compute_e(n_digits, batched_tree=True):
+    """'2.718...' with n_digits significant digits (reference compute_e output value)."""
+    L = lib()
+    init()
+    L.ntt_compute_e.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
+                                ctypes.POINTER(EStats)]
+    buf = ctypes.create_string_buffer(n_digits + 2)
+    st = EStats()
+    check(L.ntt_compute_e(n_digits, 1 if batched_tree else 0, buf, n_digits + 2, ctypes.byref(st)))
+    return buf.value.decode(), {k: getattr(st, k) for k, _ in EStats._fields_}
