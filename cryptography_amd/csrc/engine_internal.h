@@ -15,6 +15,7 @@
 #include "ntt_kernels.h"
 
 #define NTT_MAX_MARKS 40
+#include "modarith.h"
 
 namespace ntte {
 
@@ -31,6 +32,9 @@ struct Engine {
     uint64_t *d_a = nullptr, *d_b = nullptr, *d_c = nullptr;
     size_t d_a_words = 0, d_b_words = 0, d_c_words = 0;
     int threshold = 64;
+    /* largest sub-transform (log2) per pass, per transform length 2^l;
+     * 0 = default (FT_MAX_LOGS).  Set by the autotuner. */
+    uint8_t max_logs[NTT_MAX_LOG_LEN + 1] = {};
     unsigned long dispatches = 0;
     int timing = 0;
     /* timing marks: one event after each launch of the last multiply */
@@ -53,6 +57,7 @@ struct Workspace {
     size_t res_words = 0;
     uint8_t *scratch = nullptr;
     size_t scratch_bytes = 0;
+    int *flag = nullptr;          /* integrity flag for work on this workspace (null: the engine's) */
     void release()
     {
         if (res) (void)hipFree(res);
@@ -96,6 +101,7 @@ int log_len(size_t na, size_t nb);
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);
 int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hipStream_t st);
 int sync_check();
+int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int square);
 int copy_h2d(void *dst, const void *src, size_t bytes);
 int copy_d2h(void *dst, const void *src, size_t bytes);
 

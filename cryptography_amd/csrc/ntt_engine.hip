@@ -134,9 +134,12 @@ void engine_teardown()
     for (int i = 0; i < NTT_MAX_MARKS; i++) (void)hipEventDestroy(g.ev[i]);
     (void)hipStreamDestroy(g.stream);
     int thr = g.threshold;
+    uint8_t ml[NTT_MAX_LOG_LEN + 1];
+    memcpy(ml, g.max_logs, sizeof ml);
     g.~Engine();
     new (&g) Engine();
     g.threshold = thr;
+    memcpy(g.max_logs, ml, sizeof ml);
 }
 
 /* Copies ordered after everything queued on the engine stream (the stream is
@@ -163,12 +166,14 @@ int log_len(size_t na, size_t nb)
 }
 
 /* Pass sizes of a length-2^logL transform: as few passes as possible with at
- * most 2^FT_MAX_LOGS points each, balanced, the largest last (the last pass
- * is contiguous, the earlier ones read columns). */
+ * most 2^cap points each (cap from the autotuner, else FT_MAX_LOGS),
+ * balanced, the largest last (the last pass is contiguous, the earlier ones
+ * read columns). */
 PassPlan plan_passes(int logL)
 {
     PassPlan pp{};
-    pp.npass = (logL + FT_MAX_LOGS - 1) / FT_MAX_LOGS;
+    int cap = g.max_logs[logL] ? g.max_logs[logL] : FT_MAX_LOGS;
+    pp.npass = (logL + cap - 1) / cap;
     if (pp.npass < 1) pp.npass = 1;
     int base = logL / pp.npass, extra = logL % pp.npass;
     for (int i = 0; i < pp.npass; i++) pp.logS[i] = base + (i >= pp.npass - extra ? 1 : 0);
@@ -288,7 +293,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     ga.blk_per_prod = carry_blocks(max_nout);
     ga.gen = ws.scratch;
     ga.blk = ws.scratch + (size_t)nprod * ga.blk_per_prod * 4 * CW_ROUNDS * 8;
-    ga.flag = g.flag;
+    ga.flag = ws.flag ? ws.flag : g.flag;
     ga.gc = g.gc;
     hipLaunchKernelGGL(k_garner_pack, dim3(ga.blk_per_prod, nprod), dim3(256), 0, st, ga);
     HIP_TRY(mark("k_garner_pack"));
@@ -300,9 +305,10 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     return NTT_OK;
 }
 
-int enqueue_schoolbook(const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, hipStream_t st)
+int enqueue_schoolbook(const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, hipStream_t st, int *flag)
 {
-    hipLaunchKernelGGL(k_schoolbook, dim3((nprod + 3) / 4), dim3(256), 0, st, d_desc, one, nprod, g.flag);
+    hipLaunchKernelGGL(k_schoolbook, dim3((nprod + 3) / 4), dim3(256), 0, st, d_desc, one, nprod,
+                       flag ? flag : g.flag);
     HIP_TRY(mark("k_schoolbook"));
     HIP_TRY(hipGetLastError());
     return NTT_OK;
@@ -355,7 +361,7 @@ int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d
     one.na = (uint32_t)na;
     one.nb = (uint32_t)nb;
     one.n_out = (uint32_t)nout;
-    if (use_schoolbook(na, nb, nout)) return enqueue_schoolbook(nullptr, one, 1, g.stream);
+    if (use_schoolbook(na, nb, nout)) return enqueue_schoolbook(nullptr, one, 1, g.stream, nullptr);
     const int logL = log_len(na, nb);
     size_t rw, sb;
     class_needs(logL, 1, (uint32_t)nout, square, rw, sb);
@@ -423,7 +429,7 @@ int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hip
             HIP_TRY(hipGetLastError());
             rc = NTT_OK;
         } else if (c.logL == 0) {
-            rc = enqueue_schoolbook(dd, MulDesc{}, c.count, st);
+            rc = enqueue_schoolbook(dd, MulDesc{}, c.count, st, ws.flag);
         } else {
             rc = enqueue_ntt_class(ws, dd, MulDesc{}, c.count, c.logL, c.max_nout, 0, st);
         }
@@ -658,6 +664,20 @@ int ntt_batch_graph_launch(ntt_batch_graph *gr)
 }
 
 int ntt_engine_get_threshold(void) { return g.threshold; }
+
+int ntt_engine_set_pass_cap(int log_len, int cap)
+{
+    if (log_len < 1 || log_len > NTT_MAX_LOG_LEN || (cap != 0 && (cap < 4 || cap > FT_MAX_LOGS)))
+        return fail(NTT_ERR_ARG, "pass cap %d for 2^%d out of range", cap, log_len);
+    g.max_logs[log_len] = (uint8_t)cap;
+    return NTT_OK;
+}
+
+int ntt_engine_get_pass_cap(int log_len)
+{
+    if (log_len < 1 || log_len > NTT_MAX_LOG_LEN) return 0;
+    return g.max_logs[log_len] ? g.max_logs[log_len] : FT_MAX_LOGS;
+}
 void ntt_engine_set_threshold(int limbs) { g.threshold = limbs; }
 unsigned long ntt_engine_dispatch_count(void) { return g.dispatches; }
 void ntt_engine_set_timing(int on) { g.timing = on; }

@@ -124,6 +124,7 @@ struct ntt_split_tree {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     size_t n_products = 0;
+    int per_product = 0;
 };
 
 static void tree_free(ntt_split_tree *t)
@@ -138,8 +139,10 @@ static void tree_free(ntt_split_tree *t)
     delete t;
 }
 
-/* Enqueue leaves and every level on `st` (also used under graph capture). */
-static int tree_enqueue(ntt_split_tree *t, hipStream_t st)
+/* Enqueue leaves and every level on `st` (also used under graph capture).
+ * per_product: launch every product on its own (the unbatched device path,
+ * same arena and order) instead of one batch per level. */
+static int tree_enqueue(ntt_split_tree *t, hipStream_t st, int per_product)
 {
     if (t->n_leaves) {
         hipLaunchKernelGGL(k_leaves, dim3((t->n_leaves + 255) / 256), dim3(256), 0, st, t->d_leaves,
@@ -148,6 +151,15 @@ static int tree_enqueue(ntt_split_tree *t, hipStream_t st)
     }
     for (size_t h = 0; h < t->levels.size(); h++) {
         if (t->levels[h].sorted.empty()) continue;
+        if (per_product) {
+            for (const MulDesc &d : t->levels[h].sorted) {
+                /* one launch sequence per product, bounded by its n_out */
+                ntt_mul_desc one{d.a, d.b, d.c, d.add, d.na, d.nb, d.nadd, d.n_out};
+                int rc = ntt_mul_batch_device_async(&one, 1);
+                if (rc) return rc;
+            }
+            continue;
+        }
         int rc = enqueue_batch(t->ws, t->levels[h], t->d_desc + t->level_off[h], st);
         if (rc) return rc;
     }
@@ -173,6 +185,7 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
     ntt_split_tree *t = new ntt_split_tree();
     t->a = a;
     t->b = b;
+    t->per_product = use_graph == 2;
     if (b <= a) {
         TreeNode nd;
         nd.a = a;
@@ -240,14 +253,14 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
                               t->levels[h].sorted.size() * sizeof(MulDesc));
     }
     if (!rc) rc = ws_reserve(t->ws, res_words, scratch);
-    if (!rc && use_graph) {
+    if (!rc && use_graph == 1) {
         if (hipStreamSynchronize(g.stream) != hipSuccess) rc = fail(NTT_ERR_HIP, "stream sync");
         int saved = g.timing;
         g.timing = 0;
         hipError_t e = rc ? hipSuccess : hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal);
         if (!rc && e != hipSuccess) rc = fail(NTT_ERR_HIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
         if (!rc) {
-            rc = tree_enqueue(t, g.stream);
+            rc = tree_enqueue(t, g.stream, 0);
             e = hipStreamEndCapture(g.stream, &t->graph);
             if (!rc && e != hipSuccess) rc = fail(NTT_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
         }
@@ -269,11 +282,12 @@ int ntt_split_tree_run(ntt_split_tree *t)
     std::lock_guard<std::recursive_mutex> lk(g_lock);
     if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
     if (!t) return fail(NTT_ERR_ARG, "null tree");
-    g.dispatches += t->n_products;
     if (t->exec) {
+        g.dispatches += t->n_products;
         HIP_TRY(hipGraphLaunch(t->exec, g.stream));
     } else {
-        int rc = tree_enqueue(t, g.stream);
+        if (!t->per_product) g.dispatches += t->n_products;
+        int rc = tree_enqueue(t, g.stream, t->per_product);
         if (rc) return rc;
     }
     return sync_check();

@@ -298,3 +298,60 @@ def compute_e(n_digits, batched_tree=True):
     st = EStats()
     check(L.ntt_compute_e(n_digits, 1 if batched_tree else 0, buf, n_digits + 2, ctypes.byref(st)))
     return buf.value.decode(), {k: getattr(st, k) for k, _ in EStats._fields_}
+
+
+def autotune(min_log=11, max_log=27, save=None, load=None):
+    """Run (or load) the autotuner; returns (threshold, {log_len: pass_cap})."""
+    L = lib()
+    init()
+    L.ntt_engine_autotune.argtypes = [ctypes.c_int, ctypes.c_int]
+    L.ntt_engine_autotune_save.argtypes = [ctypes.c_char_p]
+    L.ntt_engine_autotune_load.argtypes = [ctypes.c_char_p]
+    L.ntt_engine_get_pass_cap.argtypes = [ctypes.c_int]
+    if load:
+        check(L.ntt_engine_autotune_load(load.encode()))
+    else:
+        check(L.ntt_engine_autotune(min_log, max_log))
+    if save:
+        check(L.ntt_engine_autotune_save(save.encode()))
+    return L.ntt_engine_get_threshold(), {l: L.ntt_engine_get_pass_cap(l) for l in range(min_log, max_log + 1)}
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("makespan_s", ctypes.c_double), ("batches", ctypes.c_size_t), ("max_inflight", ctypes.c_size_t)]
+
+
+def serve_replay(descs, arrival, hbm_budget, slots=3):
+    """Replay an arrival stream of batch descriptors; returns (t_done, stats)."""
+    L = lib()
+    init()
+    L.ntt_serve_replay.argtypes = [ctypes.POINTER(MulDesc), ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
+                                   ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                   ctypes.POINTER(ServeStats)]
+    arr = np.ascontiguousarray(arrival, dtype=np.float64)
+    out = np.zeros(len(descs), dtype=np.float64)
+    st = ServeStats()
+    check(L.ntt_serve_replay(descs, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(descs), hbm_budget,
+                             slots, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(st)))
+    return out, {"makespan_s": st.makespan_s, "batches": st.batches, "max_inflight": st.max_inflight}
+
+
+def split_tree_time(a, b, mode, runs=1):
+    """Seconds per evaluation of the tree in the given mode (1 graph, 0 eager batches, 2 per product)."""
+    import time
+    L = lib()
+    init()
+    vp = ctypes.c_void_p
+    L.ntt_split_tree_create.argtypes = [ctypes.c_long, ctypes.c_long, ctypes.c_int, ctypes.POINTER(vp)]
+    L.ntt_split_tree_run.argtypes = [vp]
+    L.ntt_split_tree_destroy.argtypes = [vp]
+    h = vp()
+    check(L.ntt_split_tree_create(a, b, mode, ctypes.byref(h)))
+    try:
+        check(L.ntt_split_tree_run(h))       # warm
+        t0 = time.perf_counter()
+        for _ in range(runs):
+            check(L.ntt_split_tree_run(h))
+        return (time.perf_counter() - t0) / runs
+    finally:
+        L.ntt_split_tree_destroy(h)

@@ -42,8 +42,10 @@ long split_terms_needed(long n_digits);
 void binary_split_batched(SplitResult *out, long a, long b);
 
 /* ---- status-returning device tree ------------------------------------------
- * create: plan the tree, allocate its device arena and (use_graph != 0)
- * capture the whole evaluation into a hipGraph.  run: evaluate and wait. */
+ * create: plan the tree and allocate its device arena.  use_graph: 1 captures
+ * the whole level-batched evaluation into one hipGraph, 0 launches the same
+ * batches eagerly, 2 launches every product on its own (the unbatched device
+ * path, for comparison).  run: evaluate and wait. */
 typedef struct ntt_split_tree ntt_split_tree;
 int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out);
 int ntt_split_tree_run(ntt_split_tree *t);

@@ -92,6 +92,19 @@ void ntt_batch_graph_destroy(ntt_batch_graph *g);
 int ntt_engine_get_threshold(void);
 void ntt_engine_set_threshold(int limbs);
 
+/* Largest sub-transform (log2, 4..10; 0 = default 10) of each pass for
+ * transforms of length 2^log_len.  Set by the autotuner. */
+int ntt_engine_set_pass_cap(int log_len, int cap);
+int ntt_engine_get_pass_cap(int log_len);
+
+/* Autotuner: times the schoolbook/NTT crossover on batches of equal-size
+ * products and every pass split for lengths 2^min_log..2^max_log (11..28),
+ * and sets the fastest.  A few seconds.  save/load keep the result in a
+ * small JSON file ({"threshold": T, "pass_cap": {"<log_len>": cap, ...}}). */
+int ntt_engine_autotune(int min_log, int max_log);
+int ntt_engine_autotune_save(const char *path);
+int ntt_engine_autotune_load(const char *path);
+
 /* Number of device multiplies issued since init (cf. ntt_mul.hip:595). */
 unsigned long ntt_engine_dispatch_count(void);
 
