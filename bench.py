@@ -1,24 +1,36 @@
-"""Benchmark: one 2^30-bit x 2^30-bit product per step on one MI355X.
+"""Benchmark of the MI355X CRT-NTT multiply engine, one GPU.
 
-Workload: two seeded uniform-random 2^30-bit integers (top bit set), 16-bit
-coefficients, 3-prime CRT, NTT length 2^27, operands already resident in HBM.
-A step is one full device product (split, 3 x (2 forward + 1 inverse) length-
-2^27 transforms, pointwise, Garner, carry); `value` is ms per product.
+Headline (the JSON line's `value`): one 2^30-bit x 2^30-bit product per step,
+16-bit coefficients, 3-prime CRT, NTT length 2^27, operands resident in HBM;
+`value` is ms per product (lower is better).  A step is the whole device
+product: split, 3 x (2 forward + 1 inverse) length-2^27 transforms, pointwise
+product, Garner CRT, carry.  The product is checked bit-exact against GMP
+mpz_mul before timing.
 
-Roofline: the NTT stages (forward passes, pointwise, inverse passes) are
-priced with the algorithmic bytes B_mul = 104 * k * L (k = 3 primes, L = 2^27:
-2 HBM passes of 16 B/coefficient per transform, 3 transforms, plus one extra
-8 B/coefficient read of the second spectrum) against 8 TB/s.  Their time is
-measured with HIP events recorded on the engine's own stream around every
-launch.  The dominant kernel is the forward column pass (k_fast_pass<LOGS, 0>);
-its per-launch algorithmic bytes are 2 operands x 3 primes x 16 B x L.
+Roofline: the NTT stages (forward passes, fused pointwise, inverse passes) are
+priced with the algorithmic bytes B_mul = 104 * k * L (k = 3 primes,
+L = 2^27: two HBM passes of 16 B per coefficient for each of the 3 transforms
+plus one 8 B read of the second spectrum) against 8 TB/s.  Kernel times come
+from HIP events recorded after every launch on the engine's own stream.  The
+dominant kernel is the forward column pass (k_fast_pass<9, 0>); per launch it
+moves 2 operands x 3 primes x 16 B x L algorithmic bytes.
 
-CPU baseline: the oracle's 3-prime NTT multiply (one OpenMP thread per prime)
-on a bounded sample (a 2^26-bit product, L = 2^23), scaled to 2^30 bits by
-L log L; GMP mpz_mul on the same sample is reported beside it.
+Other workloads (reported under "workloads"; skip with --headline-only):
+  mul22   one 2^22 x 2^22-bit product (L = 2^19): latency, and 256 of them as
+          one graph-replayed batch
+  batch   4096 products, operand bits 2^(12 + 10u) (u uniform, independent),
+          one hipGraph replay per step, vs the same products launched one by one
+  e       e to 10^8 digits with the level-batched product tree (digits checked
+          by SHA-256 against GMP), and the tree alone at 10^6 and 10^7 digits,
+          level-batched vs one launch sequence per product
+  serve   Poisson arrivals, operand bits 2^(10 + 18u), operands resident,
+          64 GB working-memory budget, swept arrival rates: products/s, p50/p99
+CPU baseline: the oracle's 3-prime NTT multiply on a 2^26-bit sample scaled
+to 2^30 bits by L log L, with GMP mpz_mul on the same sample beside it.
 """
 import argparse
 import ctypes
+import hashlib
 import json
 import os
 import sys
@@ -28,6 +40,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 K_PRIMES = 3
@@ -39,8 +52,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--bits", type=int, default=30, help="log2 of operand bits")
+    ap.add_argument("--bits", type=int, default=30, help="log2 of operand bits (headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--headline-only", action="store_true", help="skip the other workloads")
+    ap.add_argument("--workloads", default="mul22,batch,e,serve")
+    ap.add_argument("--e-digits", type=int, default=10 ** 8)
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the product only (for rocprofv3), print nothing else")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -59,9 +75,230 @@ def kernel_times(L):
     return out
 
 
-def cpu_baseline(sample_bits=26, target_bits=30):
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import _oracle as O
+def b_alg(logL):
+    return 104 * K_PRIMES * (1 << logL)
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ---- headline: one 2^30-bit product ----------------------------------------
+
+def headline(args, E, O):
+    L = E.lib()
+    nbits = 1 << args.bits
+    a = O.random_operand(1, nbits)
+    b = O.random_operand(2, nbits)
+    na, nb = len(a), len(b)
+    logL = E.log_len(na, nb)
+    Llen = 1 << logL
+    da, db = E.DeviceBuffer.from_array(a), E.DeviceBuffer.from_array(b)
+    dc = E.DeviceBuffer(na + nb)
+    for _ in range(max(1, args.warmup)):
+        E.mul_device(dc, da, na, db, nb)
+    if not np.array_equal(dc.download(), O.gmp_mul(a, b)):
+        raise SystemExit("bench.py: GPU product differs from GMP")
+    if args.profile_only:
+        for _ in range(args.steps):
+            E.mul_device(dc, da, na, db, nb, sync=False)
+        E.engine.check(L.ntt_engine_sync())
+        return {"profile_only": True, "steps": args.steps, "transform_len": Llen}
+    E.engine.check(L.ntt_engine_sync())
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        E.mul_device(dc, da, na, db, nb, sync=False)
+    E.engine.check(L.ntt_engine_sync())
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    # per-launch event timings on the engine stream (separate, instrumented steps)
+    L.ntt_engine_set_timing(1)
+    ntt_ms, tot_ms, per = [], [], {}
+    for _ in range(max(3, args.steps // 2)):
+        E.mul_device(dc, da, na, db, nb)
+        ntt_ms.append(L.ntt_engine_last_ntt_ms())
+        tot_ms.append(L.ntt_engine_last_total_ms())
+        for i, (nm, ms) in enumerate(kernel_times(L)):
+            per.setdefault(f"{i}:{nm}", []).append(ms)
+    L.ntt_engine_set_timing(0)
+    for _ in range(2):
+        E.sqr_device(dc, da, na)
+    if not np.array_equal(dc.download(), O.gmp_mul(a, a)):
+        raise SystemExit("bench.py: GPU square differs from GMP")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        E.sqr_device(dc, da, na, sync=False)
+    E.engine.check(L.ntt_engine_sync())
+    sq_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    for x in (da, db, dc):
+        x.close()
+    t_ntt = float(np.median(ntt_ms))
+    kern = {k: float(np.median(v)) for k, v in per.items()}
+    # the dominant kernel: the forward pass that reads residues (the second launch)
+    col = [v for k, v in kern.items() if k.endswith(FWD)]
+    fwd_col = col[-1] if col else 0.0
+    fwd_bytes = 2 * K_PRIMES * 16 * Llen
+    roof = {"bound": "hbm", "kernel": "k_fast_pass<9, 0> (forward column pass)",
+            "achieved": fwd_bytes / (fwd_col * 1e-3) / 1e9 if fwd_col else None, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "traffic": None, "launch_ms": fwd_col,
+            "ntt_stages_ms": t_ntt, "ntt_stages_achieved": b_alg(logL) / (t_ntt * 1e-3) / 1e9,
+            "b_alg_bytes": b_alg(logL)}
+    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS if roof["achieved"] else None
+    roof["ntt_stages_frac"] = roof["ntt_stages_achieved"] / HBM_PEAK_GBS
+    return {"ms_per_step": ms_per_step, "square_ms": sq_ms, "logL": logL, "Llen": Llen, "nbits": nbits,
+            "kernels_ms": kern, "total_event_ms": float(np.median(tot_ms)), "roofline": roof}
+
+
+# ---- config 1: 2^22-bit products -------------------------------------------
+
+def workload_mul22(E, O):
+    from cryptography_amd import engine as EN
+    a = O.random_operand(1, 1 << 22)
+    b = O.random_operand(2, 1 << 22)
+    na, nb = len(a), len(b)
+    da, db = E.DeviceBuffer.from_array(a), E.DeviceBuffer.from_array(b)
+    dc = E.DeviceBuffer(na + nb)
+    E.mul_device(dc, da, na, db, nb)
+    want = O.gmp_mul(a, b)
+    ok = bool(np.array_equal(dc.download(), want))
+    reps = 50
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        E.mul_device(dc, da, na, db, nb)          # synchronous: latency per product
+    lat_ms = (time.perf_counter() - t0) * 1e3 / reps
+    n = 256                                        # 256 copies as one graph-replayed batch
+    pool = EN.DevicePool(n * (na + nb) + 16)
+    items, outs = [], []
+    for i in range(n):
+        oc = pool.take(na + nb)
+        outs.append(oc)
+        items.append((da.ptr, na, db.ptr, nb, pool.ptr(oc), na + nb, None, 0))
+    g = EN.BatchGraph(EN.make_descs(items))
+    g.launch()
+    EN.sync()
+    ok = ok and bool(np.array_equal(pool.download(outs[-1], na + nb), want))
+    t0 = time.perf_counter()
+    for _ in range(5):
+        g.launch()
+    EN.sync()
+    t = (time.perf_counter() - t0) / 5
+    g.close()
+    pool.close()
+    for x in (da, db, dc):
+        x.close()
+    logL = E.log_len(na, nb)
+    return {"bit_exact_vs_gmp": ok, "latency_ms": lat_ms, "transform_len": 1 << logL,
+            "batch256_products_per_s": n / t, "batch256_ms": t * 1e3,
+            "batch256_ntt_alg_gbs": n * b_alg(logL) / t / 1e9}
+
+
+# ---- config 4: 4096 mixed products in one graph ----------------------------
+
+def workload_batch(E, O):
+    from cryptography_amd import engine as EN
+    rng = np.random.default_rng(4096)
+    n = 4096
+    bits = np.floor(2.0 ** (12 + 10 * rng.random((n, 2)))).astype(np.int64)
+    la, lb = (bits[:, 0] + 63) // 64, (bits[:, 1] + 63) // 64
+    pool = EN.DevicePool(int((la + lb).sum() * 2) + 16 * n)
+    items, host = [], []
+    for i in range(n):
+        a = O.random_operand(10_000 + 2 * i, int(bits[i, 0]))
+        b = O.random_operand(10_000 + 2 * i + 1, int(bits[i, 1]))
+        oa, ob, oc = pool.take(len(a)), pool.take(len(b)), pool.take(len(a) + len(b))
+        pool.upload(oa, a)
+        pool.upload(ob, b)
+        items.append((pool.ptr(oa), len(a), pool.ptr(ob), len(b), pool.ptr(oc), len(a) + len(b), None, 0))
+        host.append((a, b, oc))
+    descs = EN.make_descs(items)
+    t0 = time.perf_counter()
+    g = EN.BatchGraph(descs)
+    t_capture = time.perf_counter() - t0
+    g.launch()
+    EN.sync()
+    bad = sum(1 for (a, b, oc) in host
+              if not np.array_equal(pool.download(oc, len(a) + len(b)), O.gmp_mul(a, b)))
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    EN.sync()
+    t = (time.perf_counter() - t0) / reps
+    # the same products, one launch sequence each (device-resident, no batching)
+    L = E.lib()
+    t0 = time.perf_counter()
+    for (pa, na, pb, nb, pc, _, _, _) in items:
+        E.engine.check(L.ntt_mul_device_async(pc, pa, na, pb, nb))
+    EN.sync()
+    t_each = time.perf_counter() - t0
+    thr = L.ntt_engine_get_threshold()
+    ntt_bytes = sum(b_alg(E.log_len(int(x), int(y))) for x, y in zip(la, lb) if max(x, y) > thr)
+    g.close()
+    pool.close()
+    return {"products": n, "all_bit_exact_vs_gmp": bad == 0, "mismatches": bad, "graph_ms": t * 1e3,
+            "products_per_s": n / t, "ntt_alg_gbs": ntt_bytes / t / 1e9, "capture_ms": t_capture * 1e3,
+            "unbatched_ms": t_each * 1e3, "speedup_vs_unbatched": t_each / t, "schoolbook_threshold_limbs": thr}
+
+
+# ---- config 3: e ------------------------------------------------------------
+
+def workload_e(E, O, digits):
+    from cryptography_amd import engine as EN
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "e_digits_gmp.json")))
+    out = {"digits": digits}
+    t0 = time.perf_counter()
+    s, st = EN.compute_e(digits)
+    out["seconds"] = time.perf_counter() - t0
+    out["phases_s"] = {k: st[k] for k in ("t_tree", "t_recip", "t_div", "t_radix", "t_total")}
+    out["n_terms"] = st["n_terms"]
+    out["device_products"] = st["products"]
+    ent = gold.get(str(digits))
+    out["sha256_matches_gmp"] = bool(ent and hashlib.sha256(s.encode()).hexdigest() == ent["sha256"])
+    comp = {}
+    for d in (10 ** 6, 10 ** 7):
+        n = O.orc().orc_split_terms_needed(d + 20)
+        t_graph = EN.split_tree_time(1, n + 1, 1, runs=3)
+        t_each = EN.split_tree_time(1, n + 1, 2, runs=1) if d <= 10 ** 6 else None
+        comp[str(d)] = {"terms": n, "batched_graph_s": t_graph, "per_product_s": t_each,
+                        "speedup": (t_each / t_graph) if t_each else None}
+    out["tree_batched_vs_per_product"] = comp
+    return out
+
+
+# ---- config 5: serving -----------------------------------------------------
+
+def workload_serve(E, O):
+    from cryptography_amd import engine as EN
+    rng = np.random.default_rng(55)
+    src_words = 1 << 24                    # operands: windows of 128 MiB of resident random limbs
+    out_words = 1 << 28                    # 2 GiB of result space, reused round-robin
+    pool = EN.DevicePool(src_words + out_words + 64)
+    src = pool.take(src_words)
+    pool.upload(src, O.splitmix(77, src_words))
+    out_base = pool.take(out_words)
+    res = {}
+    for qps in (200, 1000, 5000):
+        n = int(min(1500, max(300, qps)))
+        bits = np.floor(2.0 ** (10 + 18 * rng.random((n, 2)))).astype(np.int64)
+        items, used = [], 0
+        for i in range(n):
+            la, lb = int((bits[i, 0] + 63) // 64), int((bits[i, 1] + 63) // 64)
+            if used + la + lb > out_words:
+                used = 0
+            oa = src + int(rng.integers(0, src_words - la))
+            ob = src + int(rng.integers(0, src_words - lb))
+            items.append((pool.ptr(oa), la, pool.ptr(ob), lb, pool.ptr(out_base + used), la + lb, None, 0))
+            used += la + lb
+        arrival = np.cumsum(rng.exponential(1.0 / qps, size=n))
+        t_done, st = EN.serve_replay(EN.make_descs(items), arrival, 64 << 30, slots=3)
+        lat = (t_done - arrival) * 1e3
+        res[str(qps)] = {"requests": n, "products_per_s": n / st["makespan_s"],
+                         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                         "batches": st["batches"], "max_inflight_batches": st["max_inflight"]}
+    pool.close()
+    return {"budget_gb": 64, "slots": 3, "operand_bits": "2^(10 + 18u)", "by_qps": res}
+
+
+def cpu_baseline(O, sample_bits=26, target_bits=30):
     a = O.random_operand(101, 1 << sample_bits)
     b = O.random_operand(102, 1 << sample_bits)
     t0 = time.perf_counter()
@@ -74,8 +311,7 @@ def cpu_baseline(sample_bits=26, target_bits=30):
             "kind": "port",
             "sample": f"one 2^{sample_bits}-bit x 2^{sample_bits}-bit product (L=2^{ls}) with the oracle's "
                       f"3-prime NTT multiply, 3 OpenMP threads: {t_port * 1e3:.1f} ms; scaled by L log L x{scale:.2f}",
-            "gmp_sample_ms": t_gmp * 1e3, "gmp_scaled_ms": t_gmp * 1e3 * (2 ** (target_bits - sample_bits)) ** 1.1,
-            "nproc": os.cpu_count()}
+            "gmp_sample_ms": t_gmp * 1e3, "nproc": os.cpu_count()}
 
 
 def main():
@@ -93,94 +329,38 @@ def main():
         print(json.dumps({"copy_calibration_bytes": 2 * (1 << 30)}))
         return
     import cryptography_amd as E
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _oracle as O
-
-    L = E.lib()
     E.init()
-    nbits = 1 << args.bits
-    a = O.random_operand(1, nbits)
-    b = O.random_operand(2, nbits)
-    na, nb = len(a), len(b)
-    logL = E.log_len(na, nb)
-    Llen = 1 << logL
-    da, db = E.DeviceBuffer.from_array(a), E.DeviceBuffer.from_array(b)
-    dc = E.DeviceBuffer(na + nb)
-
-    for _ in range(args.warmup):
-        E.mul_device(dc, da, na, db, nb)
-    # correctness of what is timed: full product vs GMP (bit-exact) on the first step
-    got = dc.download()
-    want = O.gmp_mul(a, b)
-    if not np.array_equal(got, want):
-        print("bench.py: GPU product differs from GMP", file=sys.stderr)
-        sys.exit(1)
-
+    progress("headline: 2^30-bit product")
+    h = headline(args, E, O)
     if args.profile_only:
-        for _ in range(args.steps):
-            E.mul_device(dc, da, na, db, nb, sync=False)
-        E.engine.check(L.ntt_engine_sync())
-        print(json.dumps({"profile_only": True, "steps": args.steps, "transform_len": Llen}))
+        print(json.dumps(h))
         return
-
-    # timed region: K device products back to back, synchronized on both sides
-    E.engine.check(L.ntt_engine_sync())
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        E.mul_device(dc, da, na, db, nb, sync=False)
-    E.engine.check(L.ntt_engine_sync())
-    t_total = time.perf_counter() - t0
-    ms_per_step = t_total * 1e3 / args.steps
-
-    # per-launch event timings on the engine stream (separate, instrumented steps)
-    L.ntt_engine_set_timing(1)
-    ntt_ms, tot_ms, per = [], [], {}
-    for _ in range(max(3, args.steps // 2)):
-        E.mul_device(dc, da, na, db, nb)
-        ntt_ms.append(L.ntt_engine_last_ntt_ms())
-        tot_ms.append(L.ntt_engine_last_total_ms())
-        for nm, ms in kernel_times(L):
-            per.setdefault(nm, []).append(ms)
-    L.ntt_engine_set_timing(0)
-    sq_ms = []
-    for _ in range(3):
-        E.sqr_device(dc, da, na)
-    E.engine.check(L.ntt_engine_sync())
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        E.sqr_device(dc, da, na, sync=False)
-    E.engine.check(L.ntt_engine_sync())
-    sq_ms = (time.perf_counter() - t0) * 1e3 / args.steps
-
-    t_ntt = float(np.median(ntt_ms))
-    b_mul = 104 * K_PRIMES * Llen
-    n_fwd = sum(1 for nm, _ in kernel_times(L) if nm == FWD) or 1
-    fwd_avg = float(np.mean(per.get(FWD, [0.0])))
-    fwd_bytes = 2 * K_PRIMES * 16 * Llen
-    roof = {"bound": "hbm", "kernel": FWD, "achieved": fwd_bytes / (fwd_avg * 1e-3) / 1e9 if fwd_avg else None,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "traffic": None,
-            "launch_ms": fwd_avg, "launches_per_mul": n_fwd,
-            "ntt_stages_ms": t_ntt, "ntt_stages_achieved": b_mul / (t_ntt * 1e-3) / 1e9}
-    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS if roof["achieved"] else None
-    roof["ntt_stages_frac"] = roof["ntt_stages_achieved"] / HBM_PEAK_GBS
-
     line = {
         "metric": "ms per 2^30-bit mul (%HBM peak)",
-        "value": ms_per_step, "unit": "ms/mul", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "value": h["ms_per_step"], "unit": "ms/mul", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"], "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "u64", "data": "synthetic",
         "config": {"workload": f"2^{args.bits}-bit x 2^{args.bits}-bit product, 16-bit coefficients, "
-                               f"3-prime CRT, NTT length 2^{logL}, operands resident in HBM",
-                   "operand_bits": nbits, "transform_len": Llen, "primes": K_PRIMES, "seed": [1, 2]},
-        "square_ms": sq_ms,
-        "kernels_ms": {k: float(np.mean(v)) for k, v in per.items()},
-        "total_event_ms": float(np.median(tot_ms)),
-        "roofline": roof,
+                               f"3-prime CRT, NTT length 2^{h['logL']}, operands resident in HBM",
+                   "operand_bits": h["nbits"], "transform_len": h["Llen"], "primes": K_PRIMES, "seed": [1, 2]},
+        "square_ms": h["square_ms"],
+        "kernels_ms": h["kernels_ms"],
+        "total_event_ms": h["total_event_ms"],
+        "roofline": h["roofline"],
     }
+    if not args.headline_only:
+        wl = {}
+        todo = args.workloads.split(",")
+        for name, fn in (("mul22", lambda: workload_mul22(E, O)), ("batch", lambda: workload_batch(E, O)),
+                         ("e", lambda: workload_e(E, O, args.e_digits)), ("serve", lambda: workload_serve(E, O))):
+            if name in todo:
+                progress(f"workload {name}")
+                wl[name] = fn()
+        line["workloads"] = wl
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
-    for x in (da, db, dc):
-        x.close()
+        progress("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(O)
     print(json.dumps(line))
 
 

@@ -52,3 +52,12 @@ def test_compute_e_cli_output_format():
     m = re.search(r"^e = (2\.[0-9]+).*$", out, re.M)
     assert m and m.group(1) == REF["200"]["e"]
     assert out.startswith("compute_e: 200 digits, %d terms, 20 guard" % O.orc().orc_split_terms_needed(220))
+
+
+@pytest.mark.parametrize("d", [10 ** 7, 10 ** 8])
+def test_e_large_vs_gmp_hash(d):
+    """e at 10^7 and 10^8 digits: SHA-256 of "2.ddd..." against GMP (tests/golden/e_digits_gmp.json)."""
+    gold = json.load(open(os.path.join(HERE, "golden", "e_digits_gmp.json")))[str(d)]
+    s, st = EN.compute_e(d)
+    assert s[:64] == gold["head"] and s[-64:] == gold["tail"]
+    assert hashlib.sha256(s.encode()).hexdigest() == gold["sha256"]
