@@ -35,6 +35,8 @@ struct Engine {
     /* largest sub-transform (log2) per pass, per transform length 2^l;
      * 0 = default (FT_MAX_LOGS).  Set by the autotuner. */
     uint8_t max_logs[NTT_MAX_LOG_LEN + 1] = {};
+    /* most tiles one block of a forward / inverse pass works through */
+    int tile_batch = 1;
     unsigned long dispatches = 0;
     int timing = 0;
     /* timing marks: one event after each launch of the last multiply */
@@ -98,6 +100,8 @@ hipError_t mark(const char *name);
 int grow_dev(uint64_t **buf, size_t *have, size_t words);
 int ws_reserve(Workspace &ws, size_t res_words, size_t scratch_bytes);
 int log_len(size_t na, size_t nb);
+/* Row stride (words, with pads) of a class of nprod products of length 2^logL. */
+size_t row_len(int logL, uint32_t nprod);
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);
 int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hipStream_t st);
 int sync_check();

@@ -88,6 +88,44 @@ __host__ __device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b)
     return u >= Prime<PI>::P ? u - Prime<PI>::P : u;
 }
 
+/* ---- compile-time roots --------------------------------------------------
+ * Evaluated by the compiler only (static constexpr members), for the few
+ * twiddles of the smallest stages, which are then immediates in the code. */
+__host__ __device__ constexpr uint64_t c_mulmod(uint64_t a, uint64_t b, uint64_t p)
+{
+    return (uint64_t)((unsigned __int128)a * b % p);
+}
+__host__ __device__ constexpr uint64_t c_powmod(uint64_t b, uint64_t e, uint64_t p)
+{
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = c_mulmod(r, b, p);
+        b = c_mulmod(b, b, p);
+        e >>= 1;
+    }
+    return r;
+}
+/* w^E for the root w = g^((p-1)/2^LOGORD) of order 2^LOGORD (the same root as
+ * the device tables' w^(2^(30-LOGORD))), with its Shoup partner. */
+template <int PI, int LOGORD, int E>
+struct ConstTw {
+    static constexpr uint64_t P = prime(PI);
+    static constexpr uint64_t root = c_powmod(prime_gen(PI), (P - 1) >> LOGORD, P);
+    static constexpr uint64_t w = c_powmod(root, (uint64_t)E, P);
+    static constexpr uint64_t wp = (uint64_t)(((unsigned __int128)w << 64) / P);
+};
+
+/* Butterfly with twiddle 1 (either direction): (x, y) -> (x + y, x - y),
+ * [0,2p) in and out. */
+template <int PI>
+__host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
+{
+    uint64_t s = x + y;
+    uint64_t d = x - y + Prime<PI>::P2;
+    x = reduce2<PI>(s);
+    y = reduce2<PI>(d);
+}
+
 /* Forward (decimation-in-frequency) butterfly on [0,2p) values:
  * (x, y) -> (x + y, (x - y) * w), both outputs in [0, 2p). */
 template <int PI>

@@ -133,12 +133,13 @@ void engine_teardown()
     if (g.d_c) (void)hipFree(g.d_c);
     for (int i = 0; i < NTT_MAX_MARKS; i++) (void)hipEventDestroy(g.ev[i]);
     (void)hipStreamDestroy(g.stream);
-    int thr = g.threshold;
+    int thr = g.threshold, tb = g.tile_batch;
     uint8_t ml[NTT_MAX_LOG_LEN + 1];
     memcpy(ml, g.max_logs, sizeof ml);
     g.~Engine();
     new (&g) Engine();
     g.threshold = thr;
+    g.tile_batch = tb;
     memcpy(g.max_logs, ml, sizeof ml);
 }
 
@@ -180,11 +181,28 @@ PassPlan plan_passes(int logL)
     return pp;
 }
 
-/* Residue row length of a class: nprod * L rounded up to whole tiles. */
-size_t row_len(int logL, uint32_t nprod)
+/* Residues of a class row: nprod * L rounded up to whole tiles. */
+size_t row_residues(int logL, uint32_t nprod)
 {
     size_t len = ((size_t)nprod) << logL;
     return (len + FT_TILE - 1) / FT_TILE * FT_TILE;
+}
+
+/* Row padding of a length-2^logL class: a pad after every row of the first
+ * pass (stride 2^(logL - S1) residues) when that stride is 4096 or more. */
+uint32_t pad_log_for(int logL)
+{
+    const PassPlan pp = plan_passes(logL);
+    if (pp.npass < 2) return NTT_PAD_NONE;
+    int logT1 = logL - pp.logS[0];
+    return logT1 >= 12 ? (uint32_t)logT1 : NTT_PAD_NONE;
+}
+
+/* Row stride in words: the residues plus their pads. */
+size_t row_len(int logL, uint32_t nprod)
+{
+    size_t n = row_residues(logL, nprod);
+    return res_at(n, pad_log_for(logL));
 }
 
 __global__ void k_fill_zero(uint64_t *p, size_t n)
@@ -233,12 +251,13 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     const size_t L = (size_t)1 << logL;
     const size_t len = row_len(logL, nprod);
     const int nops = square ? 1 : 2;
+    const uint32_t pad_log = pad_log_for(logL);
     size_t need_res, need_scr;
     class_needs(logL, nprod, max_nout, square, need_res, need_scr);
     if (ws.res_words < need_res || ws.scratch_bytes < need_scr)
         return fail(NTT_ERR_ARG, "workspace too small for the class (internal)");
     const PassPlan pp = plan_passes(logL);
-    const size_t ntiles = len / FT_TILE;
+    const size_t ntiles = row_residues(logL, nprod) / FT_TILE;
 
     PassArgs pa{};
     pa.res = ws.res;
@@ -247,9 +266,14 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     pa.desc = d_desc;
     pa.one = one;
     pa.len = len;
+    pa.pad_log = pad_log;
     pa.nprod = nprod;
     pa.logL = logL;
     pa.square = square;
+    pa.ntiles = (uint32_t)ntiles;
+    /* several tiles per block once the grid has blocks to spare (about 4096
+     * blocks per prime row fill the chip several times over) */
+    pa.tpb = (uint32_t)std::min<size_t>(std::max<size_t>(ntiles * NTT_NPRIMES / 4096, 1), (size_t)g.tile_batch);
     for (int i = 0; i < NTT_NPRIMES; i++) {
         uint64_t p = prime(i);
         uint64_t r = (uint64_t)(((u128)1 << 64) % p);
@@ -287,6 +311,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     GarnerArgs ga{};
     ga.res = ws.res;
     ga.len = len;
+    ga.pad_log = pad_log;
     ga.logL = logL;
     ga.desc = d_desc;
     ga.one = one;
@@ -678,6 +703,13 @@ int ntt_engine_get_pass_cap(int log_len)
     if (log_len < 1 || log_len > NTT_MAX_LOG_LEN) return 0;
     return g.max_logs[log_len] ? g.max_logs[log_len] : FT_MAX_LOGS;
 }
+int ntt_engine_set_tile_batch(int tiles)
+{
+    if (tiles < 1 || tiles > 64) return fail(NTT_ERR_ARG, "tile batch %d out of range [1, 64]", tiles);
+    g.tile_batch = tiles;
+    return NTT_OK;
+}
+int ntt_engine_get_tile_batch(void) { return g.tile_batch; }
 void ntt_engine_set_threshold(int limbs) { g.threshold = limbs; }
 unsigned long ntt_engine_dispatch_count(void) { return g.dispatches; }
 void ntt_engine_set_timing(int on) { g.timing = on; }

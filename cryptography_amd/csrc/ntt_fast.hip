@@ -66,15 +66,6 @@ __device__ __forceinline__ uint32_t brev_n(uint32_t x, int bits)
     return bits ? (__brev(x) >> (32 - bits)) : 0;
 }
 
-template <int PI>
-__device__ __forceinline__ uint64_t xtw(uint64_t x, const TwPair *tw, uint32_t E)
-{
-    TwPair lo = tw[E & (NTT_TW_SIZE - 1)];
-    TwPair hi = tw[NTT_TW_SIZE + (E >> NTT_TW_LOG)];
-    x = shoup_lazy<PI>(x, lo.w, lo.wp);
-    return shoup_lazy<PI>(x, hi.w, hi.wp);
-}
-
 /* Tile geometry: G blocks x S rows x C columns, LDS index ((g*S + m) << logC) + c. */
 struct FGeom {
     uint32_t logT, logC, logG;
@@ -92,13 +83,100 @@ __device__ __forceinline__ size_t ft_addr(int LOGS, const FGeom &g, uint32_t t, 
     return ((b0 + gg) << (LOGS + g.logT)) + r + ((size_t)m << g.logT);
 }
 
+/* ---- the smallest stages with compile-time twiddles ------------------------
+ * A round whose groups are 2^RL consecutive points (group spacing d = 1) uses
+ * only w_(2hs)^e with hs <= 2^(RL-1): fixed powers of fixed roots.  They are
+ * immediates here, and e = 0 (a quarter to all of a stage's butterflies)
+ * skips the multiply.  Stage s of the round has half-size hs = 2^(RL-1-s),
+ * root order 2^(RL-s), pair distance 2^(RL-1-s) in the group. */
+template <int PI, int RL, int S, int I, bool INV>
+__device__ __forceinline__ void const_stage(uint64_t *x)
+{
+    constexpr int R = 1 << RL;
+    if constexpr (I < R) {
+        constexpr int half = R >> (S + 1);
+        if constexpr ((I & half) == 0) {
+            constexpr int e = I & (half - 1);
+            constexpr int lo = RL - S;
+            if constexpr (e == 0) {
+                bfly_one<PI>(x[I], x[I + half]);
+            } else if constexpr (INV) {
+                using T = ConstTw<PI, lo, (1 << lo) - e>;
+                dit_bfly<PI>(x[I], x[I + half], T::w, T::wp);
+            } else {
+                using T = ConstTw<PI, lo, e>;
+                dif_bfly<PI>(x[I], x[I + half], T::w, T::wp);
+            }
+        }
+        const_stage<PI, RL, S, I + 1, INV>(x);
+    }
+}
+
+/* DIF: stages top (S = 0) to bottom; DIT: bottom (S = RL-1) to top. */
+template <int PI, int RL, int S>
+__device__ __forceinline__ void const_dif(uint64_t *x)
+{
+    if constexpr (S < RL) {
+        const_stage<PI, RL, S, 0, false>(x);
+        const_dif<PI, RL, S + 1>(x);
+    }
+}
+template <int PI, int RL, int S>
+__device__ __forceinline__ void const_dit(uint64_t *x)
+{
+    if constexpr (S >= 0) {
+        const_stage<PI, RL, S, 0, true>(x);
+        const_dit<PI, RL, S - 1>(x);
+    }
+}
+
+/* ---- cross twiddles in the last forward / first inverse round -------------
+ * The four-step cross twiddle of row m, column r of a pass is
+ * w_N^(r*bitrev_S(m)).  The smallest-stage round holds, per thread, the 2^RL
+ * consecutive rows m0 + i (m0 a multiple of 2^RL) of one column, and
+ * bitrev_S(m0 + i) = bitrev_S(m0) + bitrev_RL(i) * 2^(LOGS-RL), so its
+ * twiddles are A * B^bitrev_RL(i) with A = w_N^(r*bitrev_S(m0)) and
+ * B = w_N^(r*2^(LOGS-RL)): two table look-ups (lo and hi part each) and a
+ * chain of Montgomery products instead of 2^(RL+1) random look-ups. */
+struct XtwCtx {
+    const TwPair *tw;     /* this prime's two-level table */
+    uint32_t col0;        /* first column of the tile */
+    uint32_t shift;       /* NTT_ROOT_LOG - log2(N): exponents of the 2^30 root */
+};
+
+/* w^E * 2^64 mod p (Montgomery form), in [0, 2p), for E < 2^30. */
+template <int PI>
+__device__ __forceinline__ uint64_t tw_mont(const TwPair *tw, uint32_t E)
+{
+    TwPair lo = tw[E & (NTT_TW_SIZE - 1)];
+    TwPair hi = tw[NTT_TW_SIZE + (E >> NTT_TW_LOG)];
+    /* lo.w * 2^64 mod p = -lo.wp * p mod 2^64 (the Shoup remainder) */
+    uint64_t lo_m = (uint64_t)0 - lo.wp * Prime<PI>::P;
+    return shoup_lazy<PI>(lo_m, hi.w, hi.wp);
+}
+
+/* x[i] *= w^(EA + bitrev_RL(i) * EB) for i < 2^RL (exponents mod 2^30). */
+template <int PI, int RL>
+__device__ __forceinline__ void chain_twiddle(uint64_t *x, const TwPair *tw, uint32_t EA, uint32_t EB)
+{
+    constexpr int R = 1 << RL;
+    uint64_t t = tw_mont<PI>(tw, EA & ((1u << NTT_ROOT_LOG) - 1));
+    const uint64_t b = tw_mont<PI>(tw, EB & ((1u << NTT_ROOT_LOG) - 1));
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int i = RL == 3 ? ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1) : RL == 2 ? ((k & 1) << 1) | (k >> 1) : k;
+        x[i] = mont_mul<PI>(x[i], t);
+        if (k + 1 < R) t = mont_mul<PI>(t, b);
+    }
+}
+
 /* ---- radix-R rounds over the LDS tile -------------------------------------
  * A DIF round with top stage half-size h = 2^LH and R = 2^RL covers stages
  * h, h/2, .., h/2^(RL-1): the groups {m0 + i*d}, d = h >> (RL-1),
  * m0 = blk*2h + j, j < d. */
 
-template <int PI, int LOGS, int LH, int RL>
-__device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint32_t logC)
+template <int PI, int LOGS, int LH, int RL, bool XT>
+__device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint32_t logC, const XtwCtx &xc)
 {
     constexpr int R = 1 << RL;
     constexpr int LD = LH - (RL - 1);
@@ -116,16 +194,26 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
         uint64_t x[R];
 #pragma unroll
         for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
+        if constexpr (LD == 0) {
+            const_dif<PI, RL, 0>(x);
+            if constexpr (XT) {
+                /* the outputs of rows m0 + i (m0 = blk << RL) of column col0 + c */
+                const uint32_t r = xc.col0 + c;
+                chain_twiddle<PI, RL>(x, xc.tw, (r * brev_n(m0, LOGS)) << xc.shift,
+                                      (r << (LOGS - RL)) << xc.shift);
+            }
+        } else {
 #pragma unroll
-        for (int s = 0; s < RL; s++) {
-            const int half = R >> (s + 1);          /* pair distance in group index */
-            const int lhs = LH - s;                 /* log2 of this stage's half-size */
+            for (int s = 0; s < RL; s++) {
+                const int half = R >> (s + 1);          /* pair distance in group index */
+                const int lhs = LH - s;                 /* log2 of this stage's half-size */
 #pragma unroll
-            for (int i = 0; i < R; i++) {
-                if (i & half) continue;
-                uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);   /* (m mod hs) */
-                TwPair w = twf[(1u << lhs) - 1 + e];
-                dif_bfly<PI>(x[i], x[i + half], w.w, w.wp);
+                for (int i = 0; i < R; i++) {
+                    if (i & half) continue;
+                    uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);   /* (m mod hs) */
+                    TwPair w = twf[(1u << lhs) - 1 + e];
+                    dif_bfly<PI>(x[i], x[i + half], w.w, w.wp);
+                }
             }
         }
 #pragma unroll
@@ -133,8 +221,8 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
     }
 }
 
-template <int PI, int LOGS, int LH, int RL>
-__device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint32_t logC)
+template <int PI, int LOGS, int LH, int RL, bool XT>
+__device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint32_t logC, const XtwCtx &xc)
 {
     constexpr int R = 1 << RL;
     constexpr int LD = LH - (RL - 1);
@@ -152,16 +240,26 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
         uint64_t x[R];
 #pragma unroll
         for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
+        if constexpr (LD == 0) {
+            if constexpr (XT) {
+                /* inputs of rows m0 + i of column col0 + c: inverse twiddles */
+                const uint32_t r = xc.col0 + c;
+                chain_twiddle<PI, RL>(x, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift),
+                                      0u - ((r << (LOGS - RL)) << xc.shift));
+            }
+            const_dit<PI, RL, RL - 1>(x);
+        } else {
 #pragma unroll
-        for (int s = RL - 1; s >= 0; s--) {
-            const int half = R >> (s + 1);
-            const int lhs = LH - s;
+            for (int s = RL - 1; s >= 0; s--) {
+                const int half = R >> (s + 1);
+                const int lhs = LH - s;
 #pragma unroll
-            for (int i = 0; i < R; i++) {
-                if (i & half) continue;
-                uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
-                TwPair w = twi[(1u << lhs) - 1 + e];
-                dit_bfly<PI>(x[i], x[i + half], w.w, w.wp);
+                for (int i = 0; i < R; i++) {
+                    if (i & half) continue;
+                    uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
+                    TwPair w = twi[(1u << lhs) - 1 + e];
+                    dit_bfly<PI>(x[i], x[i + half], w.w, w.wp);
+                }
             }
         }
 #pragma unroll
@@ -171,25 +269,25 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
 
 /* All DIF rounds of an S = 2^LOGS transform, top stage first, radix 8 while
  * three or more stages remain. */
-template <int PI, int LOGS, int LH>
-__device__ __forceinline__ void dif_all(uint64_t *lds, const TwPair *twf, uint32_t logC)
+template <int PI, int LOGS, int LH, bool XT = false>
+__device__ __forceinline__ void dif_all(uint64_t *lds, const TwPair *twf, uint32_t logC, const XtwCtx &xc = {})
 {
     if constexpr (LH >= 0) {
         constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
-        dif_round<PI, LOGS, LH, RL>(lds, twf, logC);
+        dif_round<PI, LOGS, LH, RL, XT>(lds, twf, logC, xc);
         __syncthreads();
-        dif_all<PI, LOGS, LH - RL>(lds, twf, logC);
+        dif_all<PI, LOGS, LH - RL, XT>(lds, twf, logC, xc);
     }
 }
 
 /* DIT rounds in the mirror order (lowest stages first). */
-template <int PI, int LOGS, int LH>
-__device__ __forceinline__ void dit_all(uint64_t *lds, const TwPair *twi, uint32_t logC)
+template <int PI, int LOGS, int LH, bool XT = false>
+__device__ __forceinline__ void dit_all(uint64_t *lds, const TwPair *twi, uint32_t logC, const XtwCtx &xc = {})
 {
     if constexpr (LH >= 0) {
         constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
-        dit_all<PI, LOGS, LH - RL>(lds, twi, logC);
-        dit_round<PI, LOGS, LH, RL>(lds, twi, logC);
+        dit_all<PI, LOGS, LH - RL, XT>(lds, twi, logC, xc);
+        dit_round<PI, LOGS, LH, RL, XT>(lds, twi, logC, xc);
         __syncthreads();
     }
 }
@@ -198,7 +296,7 @@ __device__ __forceinline__ void dit_all(uint64_t *lds, const TwPair *twi, uint32
  * stage so a round's lanes read consecutive entries: the stage with
  * half-size hs = 2^l keeps w_(2hs)^e, e < hs, at [hs - 1, 2hs - 1) of twf and
  * w_(2hs)^-e at the same place in twi (S - 1 entries each). */
-template <int PI, int LOGS>
+template <int PI, int LOGS, int DIRS>   /* DIRS: 1 forward, 2 inverse, 3 both */
 __device__ __forceinline__ void stage_twiddles(const TwPair *small_all, TwPair *twf, TwPair *twi)
 {
     const TwPair *sm = small_all + PI * (FT_SIZE / 2);
@@ -208,7 +306,8 @@ __device__ __forceinline__ void stage_twiddles(const TwPair *small_all, TwPair *
         uint32_t e = idx + 1 - (1u << l);            /* e < hs */
         /* w_(2hs)^e = w_4096^(e * 4096 / (2hs)) */
         uint32_t sh = FT_LOG - 1 - l;
-        twf[idx] = sm[e << sh];
+        if (DIRS & 1) twf[idx] = sm[e << sh];
+        if (!(DIRS & 2)) continue;
         if (e == 0) {
             twi[idx] = sm[0];
         } else {
@@ -235,56 +334,86 @@ __device__ __forceinline__ uint64_t first_in(const PassArgs &pa, int op, size_t 
     return loc < n16 ? (uint64_t)s[loc] : 0;
 }
 
-template <int PI, int LOGS, bool FIRST>
-__device__ void fwd_tile(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds, TwPair *twf)
+/* Tiles of a forward or inverse pass: block b takes tiles
+ * [b*tpb, min((b+1)*tpb, ntiles)) of its (operand, prime) row, b in XCD
+ * order.  The next tile's residues are loaded into registers while the
+ * current one is transformed in LDS, so each block keeps its loads in flight
+ * through the butterfly rounds. */
+template <int LOGS, bool FIRST>
+__device__ __forceinline__ void load_tile(const PassArgs &pa, const uint64_t *x, int op, const FGeom &g, uint32_t t,
+                                          uint64_t *v)
 {
-    FGeom g{pa.logT, pa.logC, pa.logG};
-    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
 #pragma unroll
     for (int k = 0; k < FT_PER; k++) {
-        uint32_t q = threadIdx.x + k * FTPB, m, r;
+        uint32_t q = tid + k * FTPB, m, r;
         size_t a = ft_addr(LOGS, g, t, q, m, r);
-        lds[LP(q)] = FIRST ? first_in(pa, op, a) : x[a];
+        v[k] = FIRST ? first_in(pa, op, a) : x[res_at(a, pa.pad_log)];
     }
-    __syncthreads();
-    dif_all<PI, LOGS, LOGS - 1>(lds, twf, g.logC);
-    const uint32_t logN = LOGS + g.logT;
-    const TwPair *tw = pa.tw + (size_t)PI * 2 * NTT_TW_SIZE;
+}
+
+template <int PI, int LOGS, bool FIRST>
+__device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds, const TwPair *twf)
+{
+    const FGeom g{pa.logT, pa.logC, pa.logG};
+    uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
+    const uint32_t tend = min(t + pa.tpb, pa.ntiles);
+    const uint32_t colTiles = g.logT - g.logC;
+    XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT)};
+    uint64_t v[FT_PER];
+    load_tile<LOGS, FIRST>(pa, x, op, g, t, v);
+    for (;;) {
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
 #pragma unroll
-    for (int k = 0; k < FT_PER; k++) {
-        uint32_t q = threadIdx.x + k * FTPB, m, r;
-        size_t a = ft_addr(LOGS, g, t, q, m, r);
-        uint64_t v = lds[LP(q)];
-        if (g.logT) v = xtw<PI>(v, tw, (r * brev_n(m, LOGS)) << (NTT_ROOT_LOG - logN));
-        x[a] = v;
+        for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
+        __syncthreads();
+        if (t + 1 < tend) load_tile<LOGS, FIRST>(pa, x, op, g, t + 1, v);
+        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+        dif_all<PI, LOGS, LOGS - 1, true>(lds, twf, g.logC, xc);   /* cross twiddles in the last round */
+        tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+#pragma unroll
+        for (int k = 0; k < FT_PER; k++) {
+            uint32_t q = tid + k * FTPB, m, r;
+            size_t a = ft_addr(LOGS, g, t, q, m, r);
+            x[res_at(a, pa.pad_log)] = lds[LP(q)];
+        }
+        if (++t >= tend) break;
+        __syncthreads();
     }
 }
 
 template <int PI, int LOGS>
-__device__ void inv_tile(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair *twi)
+__device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, const TwPair *twi)
 {
-    FGeom g{pa.logT, pa.logC, pa.logG};
-    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t logN = LOGS + g.logT;
-    const TwPair *tw = pa.tw + (size_t)PI * 2 * NTT_TW_SIZE;
+    const FGeom g{pa.logT, pa.logC, pa.logG};
+    uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
+    const uint32_t tend = min(t + pa.tpb, pa.ntiles);
+    const uint32_t colTiles = g.logT - g.logC;
+    XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT)};
+    uint64_t v[FT_PER];
+    load_tile<LOGS, false>(pa, x, 0, g, t, v);
+    for (;;) {
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
 #pragma unroll
-    for (int k = 0; k < FT_PER; k++) {
-        uint32_t q = threadIdx.x + k * FTPB, m, r;
-        size_t a = ft_addr(LOGS, g, t, q, m, r);
-        uint64_t v = x[a];
-        if (g.logT) {
-            uint32_t E = (r * brev_n(m, LOGS)) << (NTT_ROOT_LOG - logN);
-            v = xtw<PI>(v, tw, (0u - E) & ((1u << NTT_ROOT_LOG) - 1));
+        for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
+        __syncthreads();
+        if (t + 1 < tend) load_tile<LOGS, false>(pa, x, 0, g, t + 1, v);
+        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+        dit_all<PI, LOGS, LOGS - 1, true>(lds, twi, g.logC, xc);   /* inverse cross twiddles in the first round */
+        tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+#pragma unroll
+        for (int k = 0; k < FT_PER; k++) {
+            uint32_t q = tid + k * FTPB, m, r;
+            size_t a = ft_addr(LOGS, g, t, q, m, r);
+            x[res_at(a, pa.pad_log)] = lds[LP(q)];
         }
-        lds[LP(q)] = v;
-    }
-    __syncthreads();
-    dit_all<PI, LOGS, LOGS - 1>(lds, twi, g.logC);
-#pragma unroll
-    for (int k = 0; k < FT_PER; k++) {
-        uint32_t q = threadIdx.x + k * FTPB, m, r;
-        size_t a = ft_addr(LOGS, g, t, q, m, r);
-        x[a] = lds[LP(q)];
+        if (++t >= tend) break;
+        __syncthreads();
     }
 }
 
@@ -301,7 +430,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB, m, r;
         size_t a = ft_addr(LOGS, g, t, q, m, r);
-        lds[LP(q)] = FIRST ? first_in(pa, 0, a) : xa[a];
+        lds[LP(q)] = FIRST ? first_in(pa, 0, a) : xa[res_at(a, pa.pad_log)];
     }
     __syncthreads();
     dif_all<PI, LOGS, LOGS - 1>(lds, twf, 0);
@@ -313,7 +442,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
         for (int k = 0; k < FT_PER; k++) {
             uint32_t q = threadIdx.x + k * FTPB, m, r;
             size_t a = ft_addr(LOGS, g, t, q, m, r);
-            lds[LP(q)] = FIRST ? first_in(pa, 1, a) : xb[a];
+            lds[LP(q)] = FIRST ? first_in(pa, 1, a) : xb[res_at(a, pa.pad_log)];
         }
         __syncthreads();
         dif_all<PI, LOGS, LOGS - 1>(lds, twf, 0);
@@ -331,7 +460,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB, m, r;
         size_t a = ft_addr(LOGS, g, t, q, m, r);
-        xa[a] = lds[LP(q)];
+        xa[res_at(a, pa.pad_log)] = lds[LP(q)];
     }
 }
 
@@ -343,21 +472,25 @@ template <int PI, int LOGS, int KIND>
 __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, int op,
                                           uint64_t *lds, TwPair *twf, TwPair *twi)
 {
-    stage_twiddles<PI, LOGS>(pa.small_tw, twf, twi);
+    constexpr int DIRS = (KIND == 0 || KIND == 3) ? 1 : KIND == 1 ? 2 : 3;
+    stage_twiddles<PI, LOGS, DIRS>(pa.small_tw, twf, twi);
     __syncthreads();
-    if constexpr (KIND == 0) fwd_tile<PI, LOGS, false>(pa, xa, op, lds, twf);
-    else if constexpr (KIND == 3) fwd_tile<PI, LOGS, true>(pa, xa, op, lds, twf);
-    else if constexpr (KIND == 1) inv_tile<PI, LOGS>(pa, xa, lds, twi);
+    if constexpr (KIND == 0) fwd_tiles<PI, LOGS, false>(pa, xa, op, lds, twf);
+    else if constexpr (KIND == 3) fwd_tiles<PI, LOGS, true>(pa, xa, op, lds, twf);
+    else if constexpr (KIND == 1) inv_tiles<PI, LOGS>(pa, xa, lds, twi);
     else if constexpr (KIND == 2) mid_tile<PI, LOGS, false>(pa, xa, xb, lds, twf, twi);
     else mid_tile<PI, LOGS, true>(pa, xa, xb, lds, twf, twi);
 }
 
 template <int LOGS, int KIND>
-__global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
+__global__ void __launch_bounds__(FTPB, KIND == 1 ? 6 : 1) k_fast_pass(PassArgs pa)
 {
+    /* stage twiddles: forward only, inverse only, or both (fused passes) */
+    constexpr int NTW = (KIND == 2 || KIND == 4) ? 2 : 1;
     __shared__ uint64_t lds[FT_LDS];
-    __shared__ TwPair twf[1 << LOGS], twi[1 << LOGS];
+    __shared__ TwPair tws[NTW << LOGS];
     static_assert((1 << LOGS) <= TW_LDS_MAX, "stage twiddles exceed LDS table");
+    TwPair *twf = tws, *twi = tws + (NTW - 1) * (1 << LOGS);
     const uint32_t pi = blockIdx.y, op = blockIdx.z;
     uint64_t *xa = pa.res + ((size_t)op * NTT_NPRIMES + pi) * pa.len;
     const uint64_t *xb = pa.res + ((size_t)NTT_NPRIMES + pi) * pa.len;
@@ -370,7 +503,11 @@ __global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
  * grid.z operands for forward passes). */
 hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntiles, int nops, hipStream_t st)
 {
-    dim3 grid((unsigned)ntiles, NTT_NPRIMES, (kind == 0 || kind == 3) ? nops : 1);
+    /* forward and inverse passes: tpb tiles per block; fused passes: one */
+    const bool looped = kind == 0 || kind == 1 || kind == 3;
+    if (looped && (pa.tpb < 1 || pa.ntiles != ntiles)) return hipErrorInvalidValue;
+    const size_t nblk = looped ? (ntiles + pa.tpb - 1) / pa.tpb : ntiles;
+    dim3 grid((unsigned)nblk, NTT_NPRIMES, (kind == 0 || kind == 3) ? nops : 1);
     dim3 block(FTPB);
 #define L1(S)                                                                                  \
     case S:                                                                                    \

@@ -18,6 +18,17 @@ struct __attribute__((aligned(16))) TwPair {
 #define CW_LIMBS (64 * CW_ROUNDS)       /* limbs per wavefront */
 #define CB_LIMBS (4 * CW_LIMBS)         /* limbs per 256-thread block */
 
+/* Residue rows are padded: after every 2^pad_log residues come NTT_PAD_WORDS
+ * unused words, so the rows of the first (most strided) pass of a long
+ * transform are not a power-of-two number of bytes apart.  pad_log =
+ * NTT_PAD_NONE disables it (every index is below 2^40). */
+#define NTT_PAD_WORDS 64
+#define NTT_PAD_NONE 40
+__host__ __device__ __forceinline__ size_t res_at(size_t a, uint32_t pad_log)
+{
+    return a + ((a >> pad_log) * NTT_PAD_WORDS);
+}
+
 /* One product of a batch: c[0..n_out) = a*b (+ add).  Device pointers. */
 struct MulDesc {
     const uint64_t *a, *b;      /* operand limbs */
@@ -34,10 +45,13 @@ struct PassArgs {
     const TwPair *small_tw;     /* w_4096^e, e < 2048, per prime */
     const MulDesc *desc;        /* nprod descriptors (device), or nullptr: use one */
     MulDesc one;
-    size_t len;                 /* row length: nprod * L rounded up to a tile */
+    size_t len;                 /* row stride in words (padded) */
+    uint32_t pad_log;           /* residue index a lives at res_at(a, pad_log) */
     uint32_t nprod;
     uint32_t logL;
     uint32_t logT, logC, logG;  /* this pass: column stride, tile columns, tile blocks */
+    uint32_t ntiles;            /* tiles per (operand, prime) row */
+    uint32_t tpb;               /* forward / inverse passes: tiles per block */
     int first;                  /* read the operands' 16-bit coefficients */
     int square;                 /* b == a: one forward transform */
     uint64_t scale[3], scale_p[3];   /* 2^64 / L mod p and Shoup partners (middle pass) */
@@ -57,7 +71,8 @@ struct GarnerConsts {
 /* Garner + carry over a size class: grid.y = product. */
 struct GarnerArgs {
     const uint64_t *res;        /* res[prime][len], operand a's slot */
-    size_t len;                 /* row length nprod * L */
+    size_t len;                 /* row stride in words (padded) */
+    uint32_t pad_log;           /* residue index a lives at res_at(a, pad_log) */
     uint32_t logL;
     const MulDesc *desc;        /* or nullptr: use one */
     MulDesc one;

@@ -133,18 +133,23 @@ __device__ __forceinline__ void cla_block_scan(uint8_t *sg)
  * by the adder itself. */
 
 /* (lo, hi) = sum_{t<4} X_{4j+t} 2^(16t) for limb j, with Garner per coefficient */
-__device__ __forceinline__ void limb_sum(const uint64_t *r0, const uint64_t *r1, const uint64_t *r2, size_t j,
-                                         size_t n_coef, const GarnerConsts &gc, int &bad, uint64_t &lo,
-                                         uint64_t &hi)
+/* Residues of prime i for the product at offset off: res + i*len + res_at(off + k).
+ * The 4 coefficients of a limb never straddle a pad (pads follow every
+ * 2^pad_log >= 4096 residues, products start at multiples of 4). */
+__device__ __forceinline__ void limb_sum(const GarnerArgs &ga, size_t off, size_t j, size_t n_coef, int &bad,
+                                         uint64_t &lo, uint64_t &hi)
 {
     lo = 0;
     hi = 0;
     size_t k0 = 4 * j;
     if (k0 >= n_coef) return;
+    const size_t at = res_at(off + k0, ga.pad_log);
+    const uint64_t *r0 = ga.res + at, *r1 = ga.res + ga.len + at, *r2 = ga.res + 2 * ga.len + at;
+    const GarnerConsts &gc = ga.gc;
     uint64_t v0[4], v1[4], v2[4];
     if (k0 + 4 <= n_coef) {
-        const ulonglong2 *p0 = (const ulonglong2 *)(r0 + k0), *p1 = (const ulonglong2 *)(r1 + k0),
-                         *p2 = (const ulonglong2 *)(r2 + k0);
+        const ulonglong2 *p0 = (const ulonglong2 *)r0, *p1 = (const ulonglong2 *)r1,
+                         *p2 = (const ulonglong2 *)r2;
         ulonglong2 a0 = p0[0], a1 = p0[1], b0 = p1[0], b1 = p1[1], c0 = p2[0], c1 = p2[1];
         v0[0] = a0.x; v0[1] = a0.y; v0[2] = a1.x; v0[3] = a1.y;
         v1[0] = b0.x; v1[1] = b0.y; v1[2] = b1.x; v1[3] = b1.y;
@@ -152,9 +157,9 @@ __device__ __forceinline__ void limb_sum(const uint64_t *r0, const uint64_t *r1,
     } else {
         for (int t = 0; t < 4; t++) {
             bool in = k0 + t < n_coef;
-            v0[t] = in ? r0[k0 + t] : 0;
-            v1[t] = in ? r1[k0 + t] : 0;
-            v2[t] = in ? r2[k0 + t] : 0;
+            v0[t] = in ? r0[t] : 0;
+            v1[t] = in ? r1[t] : 0;
+            v2[t] = in ? r2[t] : 0;
         }
     }
 #pragma unroll
@@ -180,7 +185,6 @@ __global__ void __launch_bounds__(256) k_garner_pack(GarnerArgs ga)
     const MulDesc d = ga.desc ? ga.desc[prod] : ga.one;
     const size_t L = (size_t)1 << ga.logL;
     const size_t off = (size_t)prod << ga.logL;
-    const uint64_t *r0 = ga.res + off, *r1 = ga.res + ga.len + off, *r2 = ga.res + 2 * ga.len + off;
     const size_t n_out = d.n_out;
     const size_t n_coef = 4 * ((size_t)d.na + d.nb) < L ? 4 * ((size_t)d.na + d.nb) : L;
     const size_t n_span = (n_coef + 3) / 4;
@@ -192,7 +196,7 @@ __global__ void __launch_bounds__(256) k_garner_pack(GarnerArgs ga)
     uint64_t prev_hi = 0;   /* hi of limb base-1, valid in lane 0 */
     if (lane == 0 && base > 0 && base - 1 < n_span) {
         uint64_t lo;
-        limb_sum(r0, r1, r2, base - 1, n_coef, ga.gc, bad, lo, prev_hi);
+        limb_sum(ga, off, base - 1, n_coef, bad, lo, prev_hi);
         if (d.add && base - 1 < d.nadd) {
             uint64_t nl = lo + d.add[base - 1];
             prev_hi += nl < lo;
@@ -203,7 +207,7 @@ __global__ void __launch_bounds__(256) k_garner_pack(GarnerArgs ga)
     for (int e = 0; e < CW_ROUNDS; e++) {
         size_t j = base + 64 * e + lane;
         uint64_t lo = 0, hi = 0;
-        if (j < n_span) limb_sum(r0, r1, r2, j, n_coef, ga.gc, bad, lo, hi);
+        if (j < n_span) limb_sum(ga, off, j, n_coef, bad, lo, hi);
         if (d.add && j < d.nadd) {
             uint64_t nl = lo + d.add[j];
             hi += nl < lo;

@@ -45,8 +45,7 @@ size_t req_mem(const ntt_mul_desc &d)
     if (d.na == 0 || d.nb == 0) return 64;
     if (d.na <= (uint32_t)g.threshold && d.nb <= (uint32_t)g.threshold && (size_t)d.na + d.nb <= SB_MAX_OUT)
         return 64;
-    size_t L = (size_t)1 << log_len(d.na, d.nb);
-    size_t rowL = (L + FT_TILE - 1) / FT_TILE * FT_TILE;
+    size_t rowL = row_len(log_len(d.na, d.nb), 1);
     return 2 * NTT_NPRIMES * rowL * 8 + (span / CB_LIMBS + 1) * (4 * CW_ROUNDS * 8 + 1);
 }
 

@@ -317,6 +317,15 @@ def autotune(min_log=11, max_log=27, save=None, load=None):
     return L.ntt_engine_get_threshold(), {l: L.ntt_engine_get_pass_cap(l) for l in range(min_log, max_log + 1)}
 
 
+def set_tile_batch(tiles):
+    """Most tiles one block of a forward / inverse pass works through; returns the old value."""
+    L = lib()
+    L.ntt_engine_set_tile_batch.argtypes = [ctypes.c_int]
+    old = L.ntt_engine_get_tile_batch()
+    check(L.ntt_engine_set_tile_batch(tiles))
+    return old
+
+
 class ServeStats(ctypes.Structure):
     _fields_ = [("makespan_s", ctypes.c_double), ("batches", ctypes.c_size_t), ("max_inflight", ctypes.c_size_t)]
 

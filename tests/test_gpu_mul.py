@@ -140,3 +140,19 @@ def test_three_pass_transform(na, nb):
     assert E.log_len(na, nb) >= 21
     assert np.array_equal(E.mul(a, b), O.gmp_mul(a, b))
     assert np.array_equal(E.sqr(a), O.gmp_mul(a, a))
+
+
+@pytest.mark.parametrize("tiles", [1, 2, 3, 4, 8])
+def test_tile_batch_parity(tiles):
+    """Forward / inverse passes with several tiles per block (the next tile's
+    residues loaded while the current one is transformed): a 2^27-bit
+    product (L = 2^24, 4096 tiles per row, ragged for 3) and its square."""
+    E.init()
+    a = O.random_operand(71, 1 << 27)
+    b = O.random_operand(72, (1 << 27) - 4096)
+    old = E.engine.set_tile_batch(tiles)
+    try:
+        assert np.array_equal(E.mul(a, b), O.gmp_mul(a, b))
+        assert np.array_equal(E.sqr(b), O.gmp_mul(b, b))
+    finally:
+        E.engine.set_tile_batch(old)
