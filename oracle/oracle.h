@@ -11,7 +11,7 @@
  * primes and whole-limb coefficients; the algorithms are the same.
  *
  * Pinning (what checks each function independently of this file):
- *   orc_prime / orc_root      primality and exact root order, tests/test_oracle_primes.py
+ *   orc_prime / orc_root      primality and exact root order (tests/test_oracle.py)
  *   orc_mulmod / orc_shoup_*  random and corner comparison with Python ints
  *   orc_ntt_forward/inverse   O(n^2) DFT (Python ints), reference ntt_cpu.c spectra
  *                             (oracle/_ref, committed as tests/golden/ntt_spectra.json)
