@@ -126,6 +126,24 @@ __host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
     y = reduce2<PI>(d);
 }
 
+/* Montgomery product a*b*2^-64 mod p for a, b < 2p; result in [0, 2p).
+ * Every prime here is 1 mod 2^32, so -p^-1 = -1 (mod 2^32): the reduction
+ * is two 32-bit steps whose multiplier m = -t mod 2^32 costs no multiply,
+ * each adding m*p = m + m*p_hi*2^32 and dropping 32 bits.  The bound:
+ * (a*b + m1*p + m2*p*2^32) / 2^64 < (4p/2^64) p + p + p/2^32 < 2p, as
+ * 4p < 2^64 - 2^32 for these primes. */
+template <int PI>
+__host__ __device__ __forceinline__ uint64_t mont_mul_lazy(uint64_t a, uint64_t b)
+{
+    static_assert((prime(PI) & 0xffffffffull) == 1, "p must be 1 mod 2^32");
+    constexpr uint64_t PH = prime(PI) >> 32;
+    const uint64_t lo = a * b, hi = mulhi64(a, b);
+    const uint32_t l0 = (uint32_t)lo, m1 = 0u - l0;
+    const uint64_t s = (lo >> 32) + (l0 != 0) + (uint64_t)m1 * PH;     /* (t + m1 p) / 2^32 - hi 2^32 */
+    const uint32_t s0 = (uint32_t)s, m2 = 0u - s0;
+    return hi + (s >> 32) + (s0 != 0) + (uint64_t)m2 * PH;
+}
+
 /* Forward (decimation-in-frequency) butterfly on [0,2p) values:
  * (x, y) -> (x + y, (x - y) * w), both outputs in [0, 2p). */
 template <int PI>

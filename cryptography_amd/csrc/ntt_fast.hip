@@ -1,22 +1,26 @@
 /*
  * ntt_fast.hip — the transform passes of the CRT-NTT multiply, register-blocked.
  *
- * Same four-step decomposition as the generic passes in ntt_kernels.hip (pass
- * i runs S-point sub-transforms down the columns of an S x T view of the
- * current block, then multiplies by the cross twiddle w_N^(r*bitrev(m))), with
- * three changes that matter on gfx950:
+ * A four-step (here three-factor) decomposition without transposes: pass i
+ * runs S-point DIF sub-transforms down the columns of an S x T view of the
+ * current block and multiplies by the cross twiddle w_N^(r*bitrev(m)); the
+ * spectrum stays in that scrambled order, and the inverse runs the mirrored
+ * DIT passes.  What matters on gfx950:
  *   - the S-point sub-transform runs in radix-8 rounds: each thread takes 8
  *     elements from LDS, does three butterfly stages in registers and writes
  *     them back, so a 512-point transform costs 3 LDS round trips and 3
  *     barriers instead of 9;
  *   - the S/2 stage twiddles (and their inverses) are staged once per block in
- *     LDS as Shoup pairs;
+ *     LDS as Shoup pairs; the three smallest stages use compile-time twiddles;
+ *   - the cross twiddles are applied inside the last forward (first inverse)
+ *     radix round as a Montgomery chain, four table look-ups per 8 points;
  *   - the last forward pass of both operands, the pointwise product (with the
- *     1/L scale) and the first inverse pass are one kernel (k_mid_pass): both
+ *     1/L scale) and the first inverse pass are one kernel (kind 2): both
  *     spectra of a tile are formed, multiplied and transformed back without
  *     leaving the chip.
  * A tile is 4096 residues (32 KB of LDS) worked on by 512 threads; rows of a
- * size class are padded to a multiple of 4096.
+ * size class are padded to a multiple of 4096 residues, and residue index a
+ * lives at res_at(a, pad_log) (a pad after every first-pass row).
  *
  * Reference counterparts: ntt/lib/ntt_kernel.hip:625-675 (stockham_kernel),
  * :739-778 (stockham_kernel_had: Hadamard fused into the inverse's first pass),
@@ -165,8 +169,8 @@ __device__ __forceinline__ void chain_twiddle(uint64_t *x, const TwPair *tw, uin
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const int i = RL == 3 ? ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1) : RL == 2 ? ((k & 1) << 1) | (k >> 1) : k;
-        x[i] = mont_mul<PI>(x[i], t);
-        if (k + 1 < R) t = mont_mul<PI>(t, b);
+        x[i] = mont_mul_lazy<PI>(x[i], t);
+        if (k + 1 < R) t = mont_mul_lazy<PI>(t, b);
     }
 }
 
@@ -334,11 +338,59 @@ __device__ __forceinline__ uint64_t first_in(const PassArgs &pa, int op, size_t 
     return loc < n16 ? (uint64_t)s[loc] : 0;
 }
 
+/* Limb of forward-pass input elements a .. a+3 (a a multiple of 4): the
+ * operand's u64 limb a/4 of its product, or 0 past the operand. */
+__device__ __forceinline__ uint64_t first_limb(const PassArgs &pa, int op, size_t a)
+{
+    size_t prod = a >> pa.logL;
+    if (prod >= pa.nprod) return 0;
+    const MulDesc &d = pa.desc ? pa.desc[prod] : pa.one;
+    const uint64_t *s = op ? d.b : d.a;
+    size_t n = op ? d.nb : d.na;
+    size_t j = (a & (((size_t)1 << pa.logL) - 1)) >> 2;
+    return j < n ? s[j] : 0;
+}
+
+/* The first pass's tile straight from the operands into LDS.  With four or
+ * more columns per tile, four consecutive columns of a row are one operand
+ * limb: one 8-byte load gives four coefficients. */
+template <int LOGS>
+__device__ __forceinline__ void fill_first_tile(const PassArgs &pa, int op, const FGeom &g, uint32_t t, uint64_t *lds)
+{
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    if (g.logC >= 2) {
+        uint64_t limb[FT_PER / 4];
+#pragma unroll
+        for (int k = 0; k < FT_PER / 4; k++) {
+            uint32_t q = (tid + k * FTPB) << 2, m, r;
+            size_t a = ft_addr(LOGS, g, t, q, m, r);
+            limb[k] = first_limb(pa, op, a);
+        }
+#pragma unroll
+        for (int k = 0; k < FT_PER / 4; k++) {
+            uint32_t q = (tid + k * FTPB) << 2;
+#pragma unroll
+            for (int i = 0; i < 4; i++) lds[LP(q + i)] = (limb[k] >> (16 * i)) & 0xffff;
+        }
+    } else {
+        uint64_t v[FT_PER];
+#pragma unroll
+        for (int k = 0; k < FT_PER; k++) {
+            uint32_t q = tid + k * FTPB, m, r;
+            size_t a = ft_addr(LOGS, g, t, q, m, r);
+            v[k] = first_in(pa, op, a);
+        }
+#pragma unroll
+        for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
+    }
+}
+
 /* Tiles of a forward or inverse pass: block b takes tiles
  * [b*tpb, min((b+1)*tpb, ntiles)) of its (operand, prime) row, b in XCD
- * order.  The next tile's residues are loaded into registers while the
- * current one is transformed in LDS, so each block keeps its loads in flight
- * through the butterfly rounds. */
+ * order, one after the other (the stage twiddles are staged once per block).
+ * Loading the next tile into registers during the current one's rounds was
+ * measured slower: it costs the registers that let three blocks share a CU. */
 template <int LOGS, bool FIRST>
 __device__ __forceinline__ void load_tile(const PassArgs &pa, const uint64_t *x, int op, const FGeom &g, uint32_t t,
                                           uint64_t *v)
@@ -361,15 +413,19 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
     const uint32_t colTiles = g.logT - g.logC;
     XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT)};
-    uint64_t v[FT_PER];
-    load_tile<LOGS, FIRST>(pa, x, op, g, t, v);
     for (;;) {
-        uint32_t tid = threadIdx.x;
-        asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
+        uint32_t tid;
+        if constexpr (FIRST) {
+            fill_first_tile<LOGS>(pa, op, g, t, lds);
+        } else {
+            uint64_t v[FT_PER];
+            load_tile<LOGS, false>(pa, x, op, g, t, v);
+            tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
 #pragma unroll
-        for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
+            for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
+        }
         __syncthreads();
-        if (t + 1 < tend) load_tile<LOGS, FIRST>(pa, x, op, g, t + 1, v);
         xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
         dif_all<PI, LOGS, LOGS - 1, true>(lds, twf, g.logC, xc);   /* cross twiddles in the last round */
         tid = threadIdx.x;
@@ -393,15 +449,14 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, const 
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
     const uint32_t colTiles = g.logT - g.logC;
     XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT)};
-    uint64_t v[FT_PER];
-    load_tile<LOGS, false>(pa, x, 0, g, t, v);
     for (;;) {
+        uint64_t v[FT_PER];
+        load_tile<LOGS, false>(pa, x, 0, g, t, v);
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
 #pragma unroll
         for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
         __syncthreads();
-        if (t + 1 < tend) load_tile<LOGS, false>(pa, x, 0, g, t + 1, v);
         xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
         dit_all<PI, LOGS, LOGS - 1, true>(lds, twi, g.logC, xc);   /* inverse cross twiddles in the first round */
         tid = threadIdx.x;
@@ -452,7 +507,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB;
         uint64_t b = pa.square ? ra[k] : lds[LP(q)];
-        lds[LP(q)] = shoup_lazy<PI>(mont_mul<PI>(ra[k], b), s, sp);
+        lds[LP(q)] = shoup_lazy<PI>(mont_mul_lazy<PI>(ra[k], b), s, sp);
     }
     __syncthreads();
     dit_all<PI, LOGS, LOGS - 1>(lds, twi, 0);
@@ -483,7 +538,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, cons
 }
 
 template <int LOGS, int KIND>
-__global__ void __launch_bounds__(FTPB, KIND == 1 ? 6 : 1) k_fast_pass(PassArgs pa)
+__global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
 {
     /* stage twiddles: forward only, inverse only, or both (fused passes) */
     constexpr int NTW = (KIND == 2 || KIND == 4) ? 2 : 1;

@@ -97,8 +97,8 @@ void ntt_engine_set_threshold(int limbs);
 int ntt_engine_set_pass_cap(int log_len, int cap);
 int ntt_engine_get_pass_cap(int log_len);
 /* Most tiles (4096 residues each) one block of a forward or inverse pass
- * transforms in turn, loading the next while it works on the current one
- * (default 1, range 1..64; large transforms only, small ones use 1). */
+ * transforms in turn (default 1, range 1..64; applied to large transforms
+ * only, small ones use 1). */
 int ntt_engine_set_tile_batch(int tiles);
 int ntt_engine_get_tile_batch(void);
 
