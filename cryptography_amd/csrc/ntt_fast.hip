@@ -472,12 +472,44 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, const 
     }
 }
 
+/* DIF rounds of an S = 2^LOGS transform except the last (smallest) one, and
+ * DIT rounds except the first: the fused middle pass does that round itself. */
+template <int PI, int LOGS, int LH>
+__device__ __forceinline__ void dif_upper(uint64_t *lds, const TwPair *twf)
+{
+    constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
+    if constexpr (LH - RL >= 0) {
+        dif_round<PI, LOGS, LH, RL, false>(lds, twf, 0, XtwCtx{});
+        __syncthreads();
+        dif_upper<PI, LOGS, LH - RL>(lds, twf);
+    }
+}
+template <int PI, int LOGS, int LH>
+__device__ __forceinline__ void dit_upper(uint64_t *lds, const TwPair *twi)
+{
+    constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
+    if constexpr (LH - RL >= 0) {
+        dit_upper<PI, LOGS, LH - RL>(lds, twi);
+        dit_round<PI, LOGS, LH, RL, false>(lds, twi, 0, XtwCtx{});
+        __syncthreads();
+    }
+}
+
 /* Last forward pass of a and b, pointwise product (x 2^64/L via Montgomery and
- * a Shoup scale), first inverse pass; result into a's row.  T = 1 here. */
+ * a Shoup scale), first inverse pass; result into a's row.  T = 1 here.
+ * The smallest DIF round and the smallest DIT round work on the same groups
+ * of 2^RLL consecutive points, so a's spectrum is kept in registers in that
+ * round's layout, and b's last DIF round, the product and the first DIT
+ * round run on the same registers without an LDS round trip. */
 template <int PI, int LOGS, bool FIRST>
 __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, uint64_t *lds, TwPair *twf,
                          TwPair *twi)
 {
+    constexpr int RLL = ((LOGS - 1) % 3) + 1;       /* the smallest round: 2^RLL points per group */
+    constexpr int RR = 1 << RLL;
+    constexpr int NG = (FT_SIZE >> RLL) / FTPB;      /* its groups per thread */
+    constexpr int LNB = LOGS - RLL;
+    static_assert(NG * RR == FT_PER, "group layout");
     FGeom g{0, 0, pa.logG};
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     uint64_t ra[FT_PER];
@@ -488,9 +520,16 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
         lds[LP(q)] = FIRST ? first_in(pa, 0, a) : xa[res_at(a, pa.pad_log)];
     }
     __syncthreads();
-    dif_all<PI, LOGS, LOGS - 1>(lds, twf, 0);
+    dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
+    /* a's smallest round, kept in registers */
 #pragma unroll
-    for (int k = 0; k < FT_PER; k++) ra[k] = lds[LP(threadIdx.x + k * FTPB)];
+    for (int n = 0; n < NG; n++) {
+        const uint32_t gid = threadIdx.x + n * FTPB;
+        const uint32_t base = ((gid >> LNB) << LOGS) + ((gid & ((1u << LNB) - 1)) << RLL);
+#pragma unroll
+        for (int i = 0; i < RR; i++) ra[n * RR + i] = lds[LP(base + i)];
+        const_dif<PI, RLL, 0>(ra + n * RR);
+    }
     if (!pa.square) {
         __syncthreads();
 #pragma unroll
@@ -500,17 +539,30 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
             lds[LP(q)] = FIRST ? first_in(pa, 1, a) : xb[res_at(a, pa.pad_log)];
         }
         __syncthreads();
-        dif_all<PI, LOGS, LOGS - 1>(lds, twf, 0);
+        dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
     }
     const uint64_t s = pa.scale[PI], sp = pa.scale_p[PI];
 #pragma unroll
-    for (int k = 0; k < FT_PER; k++) {
-        uint32_t q = threadIdx.x + k * FTPB;
-        uint64_t b = pa.square ? ra[k] : lds[LP(q)];
-        lds[LP(q)] = shoup_lazy<PI>(mont_mul_lazy<PI>(ra[k], b), s, sp);
+    for (int n = 0; n < NG; n++) {
+        const uint32_t gid = threadIdx.x + n * FTPB;
+        const uint32_t base = ((gid >> LNB) << LOGS) + ((gid & ((1u << LNB) - 1)) << RLL);
+        uint64_t x[RR];
+        if (pa.square) {
+#pragma unroll
+            for (int i = 0; i < RR; i++) x[i] = ra[n * RR + i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < RR; i++) x[i] = lds[LP(base + i)];
+            const_dif<PI, RLL, 0>(x);
+        }
+#pragma unroll
+        for (int i = 0; i < RR; i++) x[i] = shoup_lazy<PI>(mont_mul_lazy<PI>(ra[n * RR + i], x[i]), s, sp);
+        const_dit<PI, RLL, RLL - 1>(x);
+#pragma unroll
+        for (int i = 0; i < RR; i++) lds[LP(base + i)] = x[i];
     }
     __syncthreads();
-    dit_all<PI, LOGS, LOGS - 1>(lds, twi, 0);
+    dit_upper<PI, LOGS, LOGS - 1>(lds, twi);
 #pragma unroll
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB, m, r;
