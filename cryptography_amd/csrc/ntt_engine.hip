@@ -304,7 +304,8 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         done -= pp.logS[i];
         geom(i, done);
         pa.first = 0;
-        HIP_TRY(launch_fast_pass(pp.logS[i], 1, pa, ntiles, nops, st));
+        const int kind = (pp.logS[i] >= 3 && pa.logC + pp.logS[i] == 12) ? 5 : 1;
+        HIP_TRY(launch_fast_pass(pp.logS[i], kind, pa, ntiles, nops, st));
         HIP_TRY(mark("k_fast_pass<inv>"));
     }
     g.ntt_mark = g.n_marks - 1;

@@ -441,7 +441,13 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
     }
 }
 
-template <int PI, int LOGS>
+/* LOADCHAIN (tiles of one block, C * S = 4096): thread tid loads rows
+ * m0 + k * 2^(LOGS-3), k < 8, of column c (m0 = tid >> logC < 2^(LOGS-3)),
+ * and bitrev(m0 + k * 2^(LOGS-3)) = bitrev(m0) + bitrev3(k), so the inverse
+ * cross twiddles are applied as a chain right after the load and the DIT
+ * rounds carry no twiddle code (fewer live registers).  Otherwise they are
+ * applied in the first DIT round. */
+template <int PI, int LOGS, bool LOADCHAIN>
 __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, const TwPair *twi)
 {
     const FGeom g{pa.logT, pa.logC, pa.logG};
@@ -454,11 +460,16 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, const 
         load_tile<LOGS, false>(pa, x, 0, g, t, v);
         uint32_t tid = threadIdx.x;
         asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
+        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+        if constexpr (LOADCHAIN) {
+            const uint32_t r = xc.col0 + (tid & ((1u << g.logC) - 1)), m0 = tid >> g.logC;
+            chain_twiddle<PI, 3>(v, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift), 0u - (r << xc.shift));
+        }
 #pragma unroll
         for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
         __syncthreads();
-        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
-        dit_all<PI, LOGS, LOGS - 1, true>(lds, twi, g.logC, xc);   /* inverse cross twiddles in the first round */
+        /* otherwise the inverse cross twiddles go into the first round */
+        dit_all<PI, LOGS, LOGS - 1, !LOADCHAIN>(lds, twi, g.logC, xc);
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
 #pragma unroll
@@ -574,17 +585,19 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 #define TW_LDS_MAX 1024  /* stage-twiddle entries per direction held in LDS (S - 1) */
 
 /* kind 0: forward pass, 1: inverse pass, 2: fused middle pass,
- * 3: first forward pass (reads the operands), 4: single fused pass. */
+ * 3: first forward pass (reads the operands), 4: single fused pass,
+ * 5: inverse pass with the cross twiddles applied at load (C * S = 4096). */
 template <int PI, int LOGS, int KIND>
 __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, int op,
                                           uint64_t *lds, TwPair *twf, TwPair *twi)
 {
-    constexpr int DIRS = (KIND == 0 || KIND == 3) ? 1 : KIND == 1 ? 2 : 3;
+    constexpr int DIRS = (KIND == 0 || KIND == 3) ? 1 : (KIND == 1 || KIND == 5) ? 2 : 3;
     stage_twiddles<PI, LOGS, DIRS>(pa.small_tw, twf, twi);
     __syncthreads();
     if constexpr (KIND == 0) fwd_tiles<PI, LOGS, false>(pa, xa, op, lds, twf);
     else if constexpr (KIND == 3) fwd_tiles<PI, LOGS, true>(pa, xa, op, lds, twf);
-    else if constexpr (KIND == 1) inv_tiles<PI, LOGS>(pa, xa, lds, twi);
+    else if constexpr (KIND == 1) inv_tiles<PI, LOGS, false>(pa, xa, lds, twi);
+    else if constexpr (KIND == 5) inv_tiles<PI, LOGS, true>(pa, xa, lds, twi);
     else if constexpr (KIND == 2) mid_tile<PI, LOGS, false>(pa, xa, xb, lds, twf, twi);
     else mid_tile<PI, LOGS, true>(pa, xa, xb, lds, twf, twi);
 }
@@ -611,8 +624,9 @@ __global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
 hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntiles, int nops, hipStream_t st)
 {
     /* forward and inverse passes: tpb tiles per block; fused passes: one */
-    const bool looped = kind == 0 || kind == 1 || kind == 3;
+    const bool looped = kind == 0 || kind == 1 || kind == 3 || kind == 5;
     if (looped && (pa.tpb < 1 || pa.ntiles != ntiles)) return hipErrorInvalidValue;
+    if (kind == 5 && (logS < 3 || pa.logC + logS != 12)) return hipErrorInvalidValue;
     const size_t nblk = looped ? (ntiles + pa.tpb - 1) / pa.tpb : ntiles;
     dim3 grid((unsigned)nblk, NTT_NPRIMES, (kind == 0 || kind == 3) ? nops : 1);
     dim3 block(FTPB);
@@ -623,15 +637,29 @@ hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntile
         case 1: hipLaunchKernelGGL((k_fast_pass<S, 1>), grid, block, 0, st, pa); break;        \
         case 2: hipLaunchKernelGGL((k_fast_pass<S, 2>), grid, block, 0, st, pa); break;        \
         case 3: hipLaunchKernelGGL((k_fast_pass<S, 3>), grid, block, 0, st, pa); break;        \
-        default: hipLaunchKernelGGL((k_fast_pass<S, 4>), grid, block, 0, st, pa); break;       \
+        case 4: hipLaunchKernelGGL((k_fast_pass<S, 4>), grid, block, 0, st, pa); break;        \
+        default: return hipErrorInvalidValue;                                                  \
+        }                                                                                      \
+        break;
+#define L5(S)                                                                                  \
+    case S:                                                                                    \
+        switch (kind) {                                                                        \
+        case 0: hipLaunchKernelGGL((k_fast_pass<S, 0>), grid, block, 0, st, pa); break;        \
+        case 1: hipLaunchKernelGGL((k_fast_pass<S, 1>), grid, block, 0, st, pa); break;        \
+        case 2: hipLaunchKernelGGL((k_fast_pass<S, 2>), grid, block, 0, st, pa); break;        \
+        case 3: hipLaunchKernelGGL((k_fast_pass<S, 3>), grid, block, 0, st, pa); break;        \
+        case 4: hipLaunchKernelGGL((k_fast_pass<S, 4>), grid, block, 0, st, pa); break;        \
+        case 5: hipLaunchKernelGGL((k_fast_pass<S, 5>), grid, block, 0, st, pa); break;        \
+        default: return hipErrorInvalidValue;                                                  \
         }                                                                                      \
         break;
     switch (logS) {
-        L1(1) L1(2) L1(3) L1(4) L1(5) L1(6) L1(7) L1(8) L1(9) L1(10)
+        L1(1) L1(2) L5(3) L5(4) L5(5) L5(6) L5(7) L5(8) L5(9) L5(10)
     default:
         return hipErrorInvalidValue;
     }
 #undef L1
+#undef L5
     return hipGetLastError();
 }
 
