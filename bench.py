@@ -12,8 +12,9 @@ priced with the algorithmic bytes B_mul = 104 * k * L (k = 3 primes,
 L = 2^27: two HBM passes of 16 B per coefficient for each of the 3 transforms
 plus one 8 B read of the second spectrum) against 8 TB/s.  Kernel times come
 from HIP events recorded after every launch on the engine's own stream.  The
-dominant kernel is the forward column pass (k_fast_pass<9, 0>); per launch it
-moves 2 operands x 3 primes x 16 B x L algorithmic bytes.
+dominant kernel is the launch with the longest median time; its algorithmic
+bytes per launch come from launch_table(), its HBM traffic per launch from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/).
 
 Other workloads (reported under "workloads"; skip with --headline-only):
   mul22   one 2^22 x 2^22-bit product (L = 2^19): latency, and 256 of them as
@@ -44,7 +45,6 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0
 K_PRIMES = 3
-FWD = "k_fast_pass<fwd>"     # the dominant kernel: a forward column pass over both operands
 
 
 def parse():
@@ -77,6 +77,53 @@ def kernel_times(L):
 
 def b_alg(logL):
     return 104 * K_PRIMES * (1 << logL)
+
+
+def pass_plan(L, logL):
+    """The engine's pass split of a length-2^logL transform (plan_passes in ntt_engine.hip)."""
+    L.ntt_engine_get_pass_cap.argtypes = [ctypes.c_int]
+    cap = L.ntt_engine_get_pass_cap(logL)
+    npass = max(1, -(-logL // cap))
+    base, extra = divmod(logL, npass)
+    return [base + (1 if i >= npass - extra else 0) for i in range(npass)]
+
+
+def launch_table(L, logL, na, nb):
+    """Per launch of one product (in launch order): rocprof kernel name, blocks, algorithmic bytes.
+    Algorithmic bytes: residues are 8 B, k = 3 primes; forward passes cover both operands, the first
+    one reads the operand limbs instead of residues; the fused middle pass reads both spectra and
+    writes one; inverse passes read and write one row set; Garner reads 3 residue rows and writes
+    the product limbs."""
+    Lw = 1 << logL
+    ntiles = max(1, Lw // 4096)
+    plan = pass_plan(L, logL)
+    out, done = [], 0
+    for i, s in enumerate(plan[:-1]):
+        kind = 3 if i == 0 else 0
+        byt = 8 * (na + nb) + 2 * K_PRIMES * 8 * Lw if i == 0 else 2 * K_PRIMES * 16 * Lw
+        out.append((f"k_fast_pass<{s}, {kind}>", ntiles * K_PRIMES * 2, byt))
+        done += s
+    out.append((f"k_fast_pass<{plan[-1]}, {2 if len(plan) > 1 else 4}>", ntiles * K_PRIMES, K_PRIMES * 24 * Lw))
+    for i in range(len(plan) - 2, -1, -1):
+        done -= plan[i]
+        logT = logL - done - plan[i]
+        logC = min(logT, 12 - plan[i])
+        kind = 5 if plan[i] >= 3 and logC + plan[i] == 12 else 1
+        out.append((f"k_fast_pass<{plan[i]}, {kind}>", ntiles * K_PRIMES, K_PRIMES * 16 * Lw))
+    out.append(("k_garner_pack", None, K_PRIMES * 8 * Lw + 8 * (na + nb)))
+    return out
+
+
+def profile_traffic(name, blocks):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary
+    (profiles/mul_2p30_kernels.json: FETCH_SIZE and WRITE_SIZE, scaled by a copy calibration)."""
+    path = os.path.join(ROOT, "profiles", "mul_2p30_kernels.json")
+    if not os.path.exists(path):
+        return None
+    k = json.load(open(path))["kernels"].get(f"{name}|{blocks}")
+    if not k or k.get("fetch_bytes") is None or k.get("write_bytes") is None:
+        return None
+    return k["fetch_bytes"] + k["write_bytes"]
 
 
 def progress(msg):
@@ -133,16 +180,19 @@ def headline(args, E, O):
         x.close()
     t_ntt = float(np.median(ntt_ms))
     kern = {k: float(np.median(v)) for k, v in per.items()}
-    # the dominant kernel: the forward pass that reads residues (the second launch)
-    col = [v for k, v in kern.items() if k.endswith(FWD)]
-    fwd_col = col[-1] if col else 0.0
-    fwd_bytes = 2 * K_PRIMES * 16 * Llen
-    roof = {"bound": "hbm", "kernel": "k_fast_pass<9, 0> (forward column pass)",
-            "achieved": fwd_bytes / (fwd_col * 1e-3) / 1e9 if fwd_col else None, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "traffic": None, "launch_ms": fwd_col,
+    # the dominant kernel: the launch with the longest median time
+    table = launch_table(L, logL, na, nb)
+    times = list(kern.values())
+    i_dom = int(np.argmax(times[:len(table)]))
+    name, blocks, alg = table[i_dom]
+    t_dom = times[i_dom]
+    roof = {"bound": "hbm", "kernel": f"{name} (launch {i_dom} of the product)",
+            "achieved": alg / (t_dom * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "traffic": profile_traffic(name, blocks) if (nbits == 1 << 30) else None,
+            "alg_bytes_per_launch": alg, "launch_ms": t_dom,
             "ntt_stages_ms": t_ntt, "ntt_stages_achieved": b_alg(logL) / (t_ntt * 1e-3) / 1e9,
             "b_alg_bytes": b_alg(logL)}
-    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS if roof["achieved"] else None
+    roof["frac"] = roof["achieved"] / HBM_PEAK_GBS
     roof["ntt_stages_frac"] = roof["ntt_stages_achieved"] / HBM_PEAK_GBS
     return {"ms_per_step": ms_per_step, "square_ms": sq_ms, "logL": logL, "Llen": Llen, "nbits": nbits,
             "kernels_ms": kern, "total_event_ms": float(np.median(tot_ms)), "roofline": roof}
