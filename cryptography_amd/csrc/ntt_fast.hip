@@ -87,6 +87,20 @@ __device__ __forceinline__ size_t ft_addr(int LOGS, const FGeom &g, uint32_t t, 
     return ((b0 + gg) << (LOGS + g.logT)) + r + ((size_t)m << g.logT);
 }
 
+/* Residue k of thread tid's share of tile t (q = tid + k * FTPB) sits at
+ * res_at(ft_addr(q)) = base + k * step: linear in k for every pass geometry,
+ * pads included (the pad index a >> pad_log is constant across a tile or
+ * grows with the row by one per row).  Two address computations per tile
+ * instead of one per residue. */
+template <int LOGS>
+__device__ __forceinline__ void tile_span(const PassArgs &pa, const FGeom &g, uint32_t t, uint32_t tid, size_t &base,
+                                          size_t &step)
+{
+    uint32_t m, r;
+    base = res_at(ft_addr(LOGS, g, t, tid, m, r), pa.pad_log);
+    step = res_at(ft_addr(LOGS, g, t, tid + FTPB, m, r), pa.pad_log) - base;
+}
+
 /* ---- the smallest stages with compile-time twiddles ------------------------
  * A round whose groups are 2^RL consecutive points (group spacing d = 1) uses
  * only w_(2hs)^e with hs <= 2^(RL-1): fixed powers of fixed roots.  They are
@@ -397,11 +411,17 @@ __device__ __forceinline__ void load_tile(const PassArgs &pa, const uint64_t *x,
 {
     uint32_t tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
+    if constexpr (FIRST) {
 #pragma unroll
-    for (int k = 0; k < FT_PER; k++) {
-        uint32_t q = tid + k * FTPB, m, r;
-        size_t a = ft_addr(LOGS, g, t, q, m, r);
-        v[k] = FIRST ? first_in(pa, op, a) : x[res_at(a, pa.pad_log)];
+        for (int k = 0; k < FT_PER; k++) {
+            uint32_t q = tid + k * FTPB, m, r;
+            v[k] = first_in(pa, op, ft_addr(LOGS, g, t, q, m, r));
+        }
+    } else {
+        size_t base, step;
+        tile_span<LOGS>(pa, g, t, tid, base, step);
+#pragma unroll
+        for (int k = 0; k < FT_PER; k++) v[k] = x[base + k * step];
     }
 }
 
@@ -430,11 +450,11 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
         dif_all<PI, LOGS, LOGS - 1, true>(lds, twf, g.logC, xc);   /* cross twiddles in the last round */
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
+        {
+            size_t base, step;
+            tile_span<LOGS>(pa, g, t, tid, base, step);
 #pragma unroll
-        for (int k = 0; k < FT_PER; k++) {
-            uint32_t q = tid + k * FTPB, m, r;
-            size_t a = ft_addr(LOGS, g, t, q, m, r);
-            x[res_at(a, pa.pad_log)] = lds[LP(q)];
+            for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
         }
         if (++t >= tend) break;
         __syncthreads();
@@ -472,11 +492,11 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, const 
         dit_all<PI, LOGS, LOGS - 1, !LOADCHAIN>(lds, twi, g.logC, xc);
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
+        {
+            size_t base, step;
+            tile_span<LOGS>(pa, g, t, tid, base, step);
 #pragma unroll
-        for (int k = 0; k < FT_PER; k++) {
-            uint32_t q = tid + k * FTPB, m, r;
-            size_t a = ft_addr(LOGS, g, t, q, m, r);
-            x[res_at(a, pa.pad_log)] = lds[LP(q)];
+            for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
         }
         if (++t >= tend) break;
         __syncthreads();
@@ -524,11 +544,12 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     FGeom g{0, 0, pa.logG};
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     uint64_t ra[FT_PER];
+    size_t base = 0, step = 0;
+    if constexpr (!FIRST) tile_span<LOGS>(pa, g, t, threadIdx.x, base, step);
 #pragma unroll
     for (int k = 0; k < FT_PER; k++) {
         uint32_t q = threadIdx.x + k * FTPB, m, r;
-        size_t a = ft_addr(LOGS, g, t, q, m, r);
-        lds[LP(q)] = FIRST ? first_in(pa, 0, a) : xa[res_at(a, pa.pad_log)];
+        lds[LP(q)] = FIRST ? first_in(pa, 0, ft_addr(LOGS, g, t, q, m, r)) : xa[base + k * step];
     }
     __syncthreads();
     dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
@@ -546,8 +567,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 #pragma unroll
         for (int k = 0; k < FT_PER; k++) {
             uint32_t q = threadIdx.x + k * FTPB, m, r;
-            size_t a = ft_addr(LOGS, g, t, q, m, r);
-            lds[LP(q)] = FIRST ? first_in(pa, 1, a) : xb[res_at(a, pa.pad_log)];
+            lds[LP(q)] = FIRST ? first_in(pa, 1, ft_addr(LOGS, g, t, q, m, r)) : xb[base + k * step];
         }
         __syncthreads();
         dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
@@ -574,12 +594,9 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     }
     __syncthreads();
     dit_upper<PI, LOGS, LOGS - 1>(lds, twi);
+    if constexpr (FIRST) tile_span<LOGS>(pa, g, t, threadIdx.x, base, step);
 #pragma unroll
-    for (int k = 0; k < FT_PER; k++) {
-        uint32_t q = threadIdx.x + k * FTPB, m, r;
-        size_t a = ft_addr(LOGS, g, t, q, m, r);
-        xa[res_at(a, pa.pad_log)] = lds[LP(q)];
-    }
+    for (int k = 0; k < FT_PER; k++) xa[base + k * step] = lds[LP(threadIdx.x + k * FTPB)];
 }
 
 #define TW_LDS_MAX 1024  /* stage-twiddle entries per direction held in LDS (S - 1) */
