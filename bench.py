@@ -236,9 +236,19 @@ def workload_mul22(E, O):
     for x in (da, db, dc):
         x.close()
     logL = E.log_len(na, nb)
+    # CPU beside it: GMP mpz_mul on this product (one core), and the reference's schoolbook
+    # (its path for this size, multiply.c:35-67, restated by the oracle) on a 2^18-bit sample
+    # scaled by (2^22 / 2^18)^2
+    t_gmp = O.gmp().gmpref_time_mul(O._ptr(a), na, O._ptr(b), nb, 5)
+    sa, sb = a[: na // 16].copy(), b[: nb // 16].copy()
+    t0 = time.perf_counter()
+    O.mul_schoolbook(sa, sb)
+    t_sb = (time.perf_counter() - t0) * 256
     return {"bit_exact_vs_gmp": ok, "latency_ms": lat_ms, "transform_len": 1 << logL,
             "batch256_products_per_s": n / t, "batch256_ms": t * 1e3,
-            "batch256_ntt_alg_gbs": n * b_alg(logL) / t / 1e9}
+            "batch256_ntt_alg_gbs": n * b_alg(logL) / t / 1e9,
+            "cpu_gmp_ms": t_gmp * 1e3, "cpu_schoolbook_est_ms": t_sb * 1e3,
+            "cpu_schoolbook_sample": "oracle schoolbook on 2^18 x 2^18 bits, x256"}
 
 
 # ---- config 4: 4096 mixed products in one graph ----------------------------
@@ -307,10 +317,14 @@ def workload_e(E, O, digits):
     for d in (10 ** 6, 10 ** 7):
         n = O.orc().orc_split_terms_needed(d + 20)
         t_graph = EN.split_tree_time(1, n + 1, 1, runs=3)
-        t_each = EN.split_tree_time(1, n + 1, 2, runs=1) if d <= 10 ** 6 else None
+        t_each = EN.split_tree_time(1, n + 1, 2, runs=1)
         comp[str(d)] = {"terms": n, "batched_graph_s": t_graph, "per_product_s": t_each,
-                        "speedup": (t_each / t_graph) if t_each else None}
+                        "speedup": t_each / t_graph}
     out["tree_batched_vs_per_product"] = comp
+    # CPU beside it: GMP binary splitting + division + conversion, one core, at 10^6 digits
+    t0 = time.perf_counter()
+    O.gmp_e_digits(10 ** 6)
+    out["cpu_gmp_e_1e6_s"] = time.perf_counter() - t0
     return out
 
 
