@@ -16,4 +16,5 @@ from .engine import (  # noqa: F401
     mul_device,
     sqr_device,
     log_len,
+    poly_mul,
 )

@@ -102,6 +102,14 @@ int ws_reserve(Workspace &ws, size_t res_words, size_t scratch_bytes);
 int log_len(size_t na, size_t nb);
 /* Row stride (words, with pads) of a class of nprod products of length 2^logL. */
 size_t row_len(int logL, uint32_t nprod);
+/* Residues per row (nprod * L rounded up to 4096) and the row pad exponent. */
+size_t row_residues(int logL, uint32_t nprod);
+uint32_t pad_log_for(int logL);
+/* Workspace of a size class, and its transforms (+ Garner and carry unless
+ * residues_in: then the rows hold residues on entry and on exit). */
+void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t &res_words, size_t &scratch);
+int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
+                      uint32_t max_nout, int square, hipStream_t st, int residues_in = 0);
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);
 int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hipStream_t st);
 int sync_check();

@@ -246,7 +246,7 @@ void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t
 /* Enqueue one size class: nprod products, transform length 2^logL, all on
  * `st`.  d_desc == nullptr means the single product `one`. */
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
-                      uint32_t max_nout, int square, hipStream_t st)
+                      uint32_t max_nout, int square, hipStream_t st, int residues_in)
 {
     const size_t L = (size_t)1 << logL;
     const size_t len = row_len(logL, nprod);
@@ -288,17 +288,19 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         pa.logC = pa.logT < room ? pa.logT : room;
         pa.logG = room - pa.logC;
     };
+    /* residues_in: the rows already hold the residues (polynomial products),
+     * so no pass reads the operands' limbs */
     uint32_t done = 0;
     for (int i = 0; i < pp.npass - 1; i++) {
         geom(i, done);
-        pa.first = i == 0;
-        HIP_TRY(launch_fast_pass(pp.logS[i], i == 0 ? 3 : 0, pa, ntiles, nops, st));
+        pa.first = i == 0 && !residues_in;
+        HIP_TRY(launch_fast_pass(pp.logS[i], pa.first ? 3 : 0, pa, ntiles, nops, st));
         HIP_TRY(mark("k_fast_pass<fwd>"));
         done += pp.logS[i];
     }
     geom(pp.npass - 1, done);
-    pa.first = pp.npass == 1;
-    HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], pp.npass == 1 ? 4 : 2, pa, ntiles, nops, st));
+    pa.first = pp.npass == 1 && !residues_in;
+    HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], pa.first ? 4 : 2, pa, ntiles, nops, st));
     HIP_TRY(mark("k_fast_pass<mid>"));
     for (int i = pp.npass - 2; i >= 0; i--) {
         done -= pp.logS[i];
@@ -309,6 +311,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         HIP_TRY(mark("k_fast_pass<inv>"));
     }
     g.ntt_mark = g.n_marks - 1;
+    if (residues_in) return NTT_OK;      /* the caller reconstructs from the residues */
     GarnerArgs ga{};
     ga.res = ws.res;
     ga.len = len;

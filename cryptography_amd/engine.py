@@ -317,6 +317,20 @@ def autotune(min_log=11, max_log=27, save=None, load=None):
     return L.ntt_engine_get_threshold(), {l: L.ntt_engine_get_pass_cap(l) for l in range(min_log, max_log + 1)}
 
 
+def poly_mul(f, g, negacyclic=False):
+    """f * g mod (X^n - 1) (or X^n + 1): (n, 3) uint64 array, row k = coefficient k as a
+    little-endian 192-bit residue modulo p0 p1 p2 (ntt_poly.h)."""
+    L = lib()
+    init()
+    f, g = _as_u64(f), _as_u64(g)
+    if len(f) != len(g):
+        raise ValueError("f and g must have the same length")
+    L.ntt_poly_mul.argtypes = [_u64p, _u64p, _u64p, ctypes.c_size_t, ctypes.c_int]
+    out = np.zeros(3 * len(f), dtype=np.uint64)
+    check(L.ntt_poly_mul(_p(out), _p(f), _p(g), len(f), 1 if negacyclic else 0))
+    return out.reshape(-1, 3)
+
+
 def set_tile_batch(tiles):
     """Most tiles one block of a forward / inverse pass works through; returns the old value."""
     L = lib()
