@@ -136,29 +136,49 @@ __global__ void k_fill(uint64_t *z, size_t n, uint64_t v)
     if (j < n) z[j] = v;
 }
 
+/* A leaf of the conversion tree: m <= LEAF_DIGITS digits of the fraction
+ * y / 2^(64 p) held in p <= LEAF_LIMBS limbs (y[0] lowest). */
+#define LEAF_STEPS 16
+#define LEAF_DIGITS (19 * LEAF_STEPS)
+#define LEAF_LIMBS 18          /* frac_limbs(LEAF_DIGITS) */
 struct DigitLeaf {
-    const uint64_t *top;   /* the two most significant limbs of the fraction: top[0] low, top[1] high */
+    const uint64_t *y;
     uint64_t pos;          /* first digit's index in the output */
-    uint32_t m;            /* digits, <= 19 */
+    uint32_t m, p;
 };
 
-/* floor(y * 10^m) for the 128-bit fraction y = (hi:lo) / 2^128, as m digits. */
-__global__ void k_leaf_digits(const DigitLeaf *leaves, size_t n, char *out)
+/* One thread per leaf: y * 10^r splits exactly into an integer part (the
+ * next r <= 19 digits) and a p-limb fraction, repeated until m digits are out. */
+__global__ void __launch_bounds__(256) k_leaf_digits(const DigitLeaf *leaves, size_t n, char *out)
 {
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const DigitLeaf L = leaves[i];
-    uint64_t lo = L.top[0], hi = L.top[1];
-    uint64_t p10 = 1;
-    for (uint32_t k = 0; k < L.m; k++) p10 *= 10;
-    /* (hi*2^64 + lo) * p10 / 2^128 = hi*p10/2^64 + lo*p10/2^128 */
-    uint64_t a_hi = __umul64hi(hi, p10), a_lo = hi * p10;
-    uint64_t b_hi = __umul64hi(lo, p10);
-    uint64_t s = a_lo + b_hi;
-    uint64_t v = a_hi + (s < a_lo);
-    for (int k = (int)L.m - 1; k >= 0; k--) {
-        out[L.pos + k] = (char)('0' + v % 10);
-        v /= 10;
+    uint64_t y[LEAF_LIMBS];
+#pragma unroll
+    for (int j = 0; j < LEAF_LIMBS; j++) y[j] = j < (int)L.p ? L.y[j] : 0;
+    uint64_t pos = L.pos;
+    for (uint32_t left = L.m; left > 0;) {
+        const uint32_t r = left < 19 ? left : 19;
+        uint64_t p10 = 1;
+        for (uint32_t k = 0; k < r; k++) p10 *= 10;
+        uint64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < LEAF_LIMBS; j++) {
+            if (j < (int)L.p) {
+                uint64_t lo = y[j] * p10, hi = __umul64hi(y[j], p10);
+                lo += carry;
+                hi += lo < carry;
+                y[j] = lo;
+                carry = hi;
+            }
+        }
+        for (int k = (int)r - 1; k >= 0; k--) {
+            out[pos + k] = (char)('0' + carry % 10);
+            carry /= 10;
+        }
+        pos += r;
+        left -= r;
     }
 }
 
@@ -439,7 +459,7 @@ extern "C" int ntt_compute_e(long n_digits, int batched_tree, char *out, size_t 
             while (!lv.empty()) {
                 nx.clear();
                 for (auto &pm : lv) {
-                    if (pm.second <= 19) continue;
+                    if (pm.second <= LEAF_DIGITS) continue;
                     size_t nl = (pm.second + 18) / 19, K = 1;
                     int j = 0;
                     while (K * 2 < nl) K *= 2, j++;
@@ -458,8 +478,12 @@ extern "C" int ntt_compute_e(long n_digits, int batched_tree, char *out, size_t 
             std::vector<RNode> next;
             std::vector<ntt_mul_desc> descs;
             for (const RNode &nd : level) {
-                if (nd.m <= 19) {
-                    leafv.push_back({nd.y + nd.p - 2, nd.pos, (uint32_t)nd.m});
+                if (nd.m <= LEAF_DIGITS) {
+                    if (nd.p > LEAF_LIMBS) {
+                        rc = fail(NTT_ERR_ARG, "leaf of %zu limbs (internal)", nd.p);
+                        break;
+                    }
+                    leafv.push_back({nd.y, nd.pos, (uint32_t)nd.m, (uint32_t)nd.p});
                     continue;
                 }
                 size_t nl = (nd.m + 18) / 19, K = 1;
