@@ -406,27 +406,30 @@ int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
 {
     static_assert(sizeof(ntt_mul_desc) == sizeof(MulDesc), "descriptor layouts differ");
-    std::vector<std::pair<int, size_t>> key(n);
+    /* class key: -1 empty operand, 0 schoolbook, else log2 of the transform
+     * length (<= NTT_MAX_LOG_LEN); a counting sort keeps the batch order
+     * within a class and costs O(n) */
+    constexpr int NK = NTT_MAX_LOG_LEN + 2;
+    std::vector<uint8_t> key(n);
+    size_t cnt[NK + 1] = {};
+    uint32_t maxo[NK] = {};
     for (size_t i = 0; i < n; i++) {
         MulDesc d;
         memcpy(&d, &h[i], sizeof d);
         int rc = check_desc(d);
         if (rc) return rc;
         int k = (d.na == 0 || d.nb == 0) ? -1 : use_schoolbook(d.na, d.nb, d.n_out) ? 0 : log_len(d.na, d.nb);
-        key[i] = {k, i};
+        key[i] = (uint8_t)(k + 1);
+        cnt[k + 2]++;
+        maxo[k + 1] = std::max(maxo[k + 1], std::max(d.n_out, d.na + d.nb));
     }
-    std::stable_sort(key.begin(), key.end());
+    for (int k = 0; k < NK; k++) cnt[k + 1] += cnt[k];   /* cnt[k] = first slot of key k */
     bp.sorted.resize(n);
     bp.classes.clear();
-    for (size_t i = 0; i < n; i++) {
-        memcpy(&bp.sorted[i], &h[key[i].second], sizeof(MulDesc));
-        const MulDesc &d = bp.sorted[i];
-        if (bp.classes.empty() || bp.classes.back().logL != key[i].first)
-            bp.classes.push_back({key[i].first, (uint32_t)i, 0, 0});
-        auto &c = bp.classes.back();
-        c.count++;
-        c.max_nout = std::max(c.max_nout, std::max(d.n_out, d.na + d.nb));
-    }
+    for (int k = 0; k < NK; k++)
+        if (cnt[k + 1] > cnt[k])
+            bp.classes.push_back({k - 1, (uint32_t)cnt[k], (uint32_t)(cnt[k + 1] - cnt[k]), maxo[k]});
+    for (size_t i = 0; i < n; i++) memcpy(&bp.sorted[cnt[key[i]]++], &h[i], sizeof(MulDesc));
     bp.res_words = bp.scratch = 0;
     for (auto &c : bp.classes) {
         if (c.logL <= 0) continue;

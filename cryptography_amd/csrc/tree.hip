@@ -192,17 +192,23 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
         nd.b = b;
         t->nodes.push_back(nd);
     } else {
+        t->nodes.reserve(2 * (size_t)((b - a + LEAF_TERMS - 1) / LEAF_TERMS) + 2);
         build_nodes(a, b, t->nodes);
     }
     /* capacities and arena offsets */
     size_t off = 0;
     int H = 0;
     for (auto &nd : t->nodes) {
-        nd.capB = cap_limbs_B(nd.a, nd.b);
-        nd.capP = nd.capB + 1;
-        if (nd.left < 0) {   /* leaf: values fit LEAF_LIMBS */
-            nd.capB = std::max<uint32_t>(std::min<uint32_t>(nd.capB, LEAF_LIMBS), 1);
-            nd.capP = std::max<uint32_t>(std::min<uint32_t>(nd.capP, LEAF_LIMBS), 1);
+        if (nd.left < 0) {
+            /* leaf: B < prod 2^bitlen(k) over at most LEAF_TERMS terms, P < 2B,
+             * both within LEAF_LIMBS */
+            uint32_t bits = 0;
+            for (long k = nd.a; k < nd.b; k++) bits += 64 - __builtin_clzl((unsigned long)k);
+            nd.capB = std::max<uint32_t>(std::min<uint32_t>(bits / 64 + 1, LEAF_LIMBS), 1);
+            nd.capP = std::max<uint32_t>(std::min<uint32_t>((bits + 1) / 64 + 1, LEAF_LIMBS), 1);
+        } else {
+            nd.capB = cap_limbs_B(nd.a, nd.b);
+            nd.capP = nd.capB + 1;
         }
         nd.offP = off;
         off += nd.capP;
