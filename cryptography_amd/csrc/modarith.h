@@ -115,8 +115,7 @@ struct ConstTw {
     static constexpr uint64_t wp = (uint64_t)(((unsigned __int128)w << 64) / P);
 };
 
-/* Butterfly with twiddle 1 (either direction): (x, y) -> (x + y, x - y),
- * [0,2p) in and out. */
+/* Forward butterfly with twiddle 1: (x, y) -> (x + y, x - y), [0,2p) in and out. */
 template <int PI>
 __host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
 {
@@ -131,7 +130,8 @@ __host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
  * is two 32-bit steps whose multiplier m = -t mod 2^32 costs no multiply,
  * each adding m*p = m + m*p_hi*2^32 and dropping 32 bits.  The bound:
  * (a*b + m1*p + m2*p*2^32) / 2^64 < (4p/2^64) p + p + p/2^32 < 2p, as
- * 4p < 2^64 - 2^32 for these primes. */
+ * 4p < 2^64 - 2^32 for these primes.  With a < 4p (an inverse-pass value)
+ * and b < 2p the same sum stays below 2p + p + p/2^32 < 4p. */
 template <int PI>
 __host__ __device__ __forceinline__ uint64_t mont_mul_lazy(uint64_t a, uint64_t b)
 {
@@ -155,16 +155,27 @@ __host__ __device__ __forceinline__ void dif_bfly(uint64_t &x, uint64_t &y, uint
     y = shoup_lazy<PI>(d, w, wp);
 }
 
-/* Inverse (decimation-in-time) butterfly on [0,2p) values:
- * (x, y) -> (x + y*w, x - y*w), both outputs in [0, 2p). */
+/* Inverse (decimation-in-time) butterfly in Harvey's [0,4p) form:
+ * (x, y) -> (x + y*w, x - y*w) for x, y in [0, 4p), both outputs in [0, 4p)
+ * (4p < 2^64 for these primes).  Only x is reduced, to [0, 2p); y*w comes
+ * from Shoup in [0, 2p) for any y.  Inverse passes therefore leave residues
+ * in [0, 4p); whatever reads them last reduces twice. */
 template <int PI>
 __host__ __device__ __forceinline__ void dit_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wp)
 {
-    uint64_t t = shoup_lazy<PI>(y, w, wp);
-    uint64_t s = x + t;
-    uint64_t d = x - t + Prime<PI>::P2;
-    x = reduce2<PI>(s);
-    y = reduce2<PI>(d);
+    const uint64_t t = shoup_lazy<PI>(y, w, wp);
+    const uint64_t u = reduce2<PI>(x);
+    x = u + t;
+    y = u - t + Prime<PI>::P2;
+}
+
+/* The same with twiddle 1: both inputs reduced to [0, 2p). */
+template <int PI>
+__host__ __device__ __forceinline__ void dit_bfly_one(uint64_t &x, uint64_t &y)
+{
+    const uint64_t u = reduce2<PI>(x), t = reduce2<PI>(y);
+    x = u + t;
+    y = u - t + Prime<PI>::P2;
 }
 
 }  // namespace nttc

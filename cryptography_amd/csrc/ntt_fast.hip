@@ -117,7 +117,8 @@ __device__ __forceinline__ void const_stage(uint64_t *x)
             constexpr int e = I & (half - 1);
             constexpr int lo = RL - S;
             if constexpr (e == 0) {
-                bfly_one<PI>(x[I], x[I + half]);
+                if constexpr (INV) dit_bfly_one<PI>(x[I], x[I + half]);
+                else bfly_one<PI>(x[I], x[I + half]);
             } else if constexpr (INV) {
                 using T = ConstTw<PI, lo, (1 << lo) - e>;
                 dit_bfly<PI>(x[I], x[I + half], T::w, T::wp);
@@ -173,7 +174,8 @@ __device__ __forceinline__ uint64_t tw_mont(const TwPair *tw, uint32_t E)
     return shoup_lazy<PI>(lo_m, hi.w, hi.wp);
 }
 
-/* x[i] *= w^(EA + bitrev_RL(i) * EB) for i < 2^RL (exponents mod 2^30). */
+/* x[i] *= w^(EA + bitrev_RL(i) * EB) for i < 2^RL (exponents mod 2^30);
+ * x in [0, 2p) gives [0, 2p), x in [0, 4p) (inverse passes) gives [0, 4p). */
 template <int PI, int RL>
 __device__ __forceinline__ void chain_twiddle(uint64_t *x, const TwPair *tw, uint32_t EA, uint32_t EB)
 {

@@ -93,9 +93,10 @@ __global__ void __launch_bounds__(256) k_twiddle_gen(TwPair *tw, uint64_t r2_0, 
 __device__ __forceinline__ uint64_t garner3(uint64_t r0, uint64_t r1, uint64_t r2, const GarnerConsts &gc, int &bad)
 {
     const uint64_t p1 = Prime<1>::P, p2 = Prime<2>::P;
-    uint64_t x0 = reduce1<0>(r0);
-    uint64_t x1 = reduce1<1>(r1);
-    uint64_t x2 = reduce1<2>(r2);
+    /* inverse-pass output in [0, 4p) */
+    uint64_t x0 = reduce1<0>(reduce2<0>(r0));
+    uint64_t x1 = reduce1<1>(reduce2<1>(r1));
+    uint64_t x2 = reduce1<2>(reduce2<2>(r2));
     uint64_t a0 = x0;
     uint64_t a0m1 = a0 >= p1 ? a0 - p1 : a0;            /* a0 < p0 < 2*p1 */
     uint64_t t1 = x1 >= a0m1 ? x1 - a0m1 : x1 + p1 - a0m1;

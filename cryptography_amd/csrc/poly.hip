@@ -61,11 +61,12 @@ __global__ void __launch_bounds__(256) k_poly_load(uint64_t *res, size_t len, ui
     res[((size_t)op * NTT_NPRIMES + pi) * len + res_at(k, pad_log)] = r;
 }
 
+/* r: inverse-pass output in [0, 4p). */
 template <int PI>
 __device__ __forceinline__ uint64_t store_one(uint64_t r, size_t k, int logn, int nega, const TwPair *tw)
 {
     if (nega) r = times_root_pow<PI>(r, tw, (0u - (uint32_t)(k << (29 - logn))) & ((1u << NTT_ROOT_LOG) - 1));
-    return reduce1<PI>(r);
+    return reduce1<PI>(reduce2<PI>(r));
 }
 
 /* out[3k .. 3k+2] = the value in [0, P) with residues r0, r1, r2: mixed radix
