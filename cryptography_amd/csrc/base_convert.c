@@ -6,7 +6,9 @@
  * Divide and conquer on the digit count: a value below 10^(2^(k+1)) splits at
  * 10^(2^k) into a quotient (the high digits) and a remainder (exactly 2^k low
  * digits, zero-padded), by bigint_div_newton with the cached reciprocal of
- * 10^(2^k); values below 10^16 are printed directly.  The reference's
+ * 10^(2^k).  Pieces below 10^2048 are converted on the host by repeated
+ * division by 10^19 (the splits there would be GPU launches of a few limbs
+ * each); values below 10^16 are printed directly.  The reference's
  * algorithm is the same divide and conquer (base_convert.c:135-259).
  */
 #include <stdint.h>
@@ -21,6 +23,7 @@
 
 #define MAX_LEVEL 40
 #define DIRECT_LEVEL 3        /* below 10^(2^(3+1)) = 10^16: one u64 */
+#define BASE_LEVEL 10         /* below 10^(2^11): converted on the host by repeated division by 10^19 */
 
 static BigInt pow10_k[MAX_LEVEL + 1], recip_k[MAX_LEVEL + 1];
 static int cache_level = -1;
@@ -72,6 +75,35 @@ static int u64_digits(uint64_t v)
     return n;
 }
 
+/* Exactly `width` digits of N (N < 10^width), zero-padded, by repeated
+ * division of the limbs by 10^19: the base case below 10^2048, where the
+ * quadratic loop is a few microseconds and the GPU splits would be dominated
+ * by their launches. */
+static void emit_small(const BigInt *N, size_t width, char *out)
+{
+    int n = N->n_limbs;
+    uint64_t *t = (uint64_t *)malloc(((size_t)n + 1) * sizeof(uint64_t));
+    if (!t) die("out of memory");
+    memcpy(t, N->limbs, (size_t)n * sizeof(uint64_t));
+    const uint64_t D = 10000000000000000000ull;   /* 10^19 */
+    size_t pos = width;
+    while (pos > 0) {
+        unsigned __int128 r = 0;
+        for (int i = n - 1; i >= 0; i--) {
+            unsigned __int128 cur = (r << 64) | t[i];
+            t[i] = (uint64_t)(cur / D);
+            r = cur % D;
+        }
+        while (n > 0 && t[n - 1] == 0) n--;
+        uint64_t d = (uint64_t)r;
+        for (int k = 0; k < 19 && pos > 0; k++) {
+            out[--pos] = (char)('0' + d % 10);
+            d /= 10;
+        }
+    }
+    free(t);
+}
+
 /* N < 10^(2^(k+1)): q = N / 10^(2^k), r = N mod 10^(2^k). */
 static void split(const BigInt *N, int k, BigInt *q, BigInt *r)
 {
@@ -87,6 +119,10 @@ static void emit_padded(const BigInt *N, int k, char *out)
 {
     if (k < DIRECT_LEVEL + 1) {
         print_u64(out, N->n_limbs ? N->limbs[0] : 0, 1 << (k + 1));
+        return;
+    }
+    if (k <= BASE_LEVEL) {
+        emit_small(N, (size_t)1 << (k + 1), out);
         return;
     }
     BigInt q = bigint_alloc(4), r = bigint_alloc(4);
@@ -105,6 +141,14 @@ static size_t emit_top(const BigInt *N, int k, char *out)
         int n = u64_digits(v);
         print_u64(out, v, n);
         return (size_t)n;
+    }
+    if (k <= BASE_LEVEL) {
+        /* N < 10^(2^(k+1)): all digits, then drop the leading zeros */
+        size_t w = (size_t)1 << (k + 1), z = 0;
+        emit_small(N, w, out);
+        while (z + 1 < w && out[z] == '0') z++;
+        memmove(out, out + z, w - z);
+        return w - z;
     }
     if (bigint_cmp(N, &pow10_k[k]) < 0) return emit_top(N, k - 1, out);
     BigInt q = bigint_alloc(4), r = bigint_alloc(4);
