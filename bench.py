@@ -26,6 +26,8 @@ Other workloads (reported under "workloads"; skip with --headline-only):
           level-batched vs one launch sequence per product
   serve   Poisson arrivals, operand bits 2^(10 + 18u), operands resident,
           64 GB working-memory budget, swept arrival rates: products/s, p50/p99
+  convert bigint_to_decimal of a 10^6-digit integer and newton_reciprocal of a
+          2^22-bit integer (the reference's base_convert.h / newton.h entry points)
 CPU baseline: the oracle's 3-prime NTT multiply on a 2^26-bit sample scaled
 to 2^30 bits by L log L, with GMP mpz_mul on the same sample beside it.
 """
@@ -55,7 +57,7 @@ def parse():
     ap.add_argument("--bits", type=int, default=30, help="log2 of operand bits (headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--headline-only", action="store_true", help="skip the other workloads")
-    ap.add_argument("--workloads", default="mul22,batch,e,serve")
+    ap.add_argument("--workloads", default="mul22,batch,e,serve,convert")
     ap.add_argument("--e-digits", type=int, default=10 ** 8)
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the product only (for rocprofv3), print nothing else")
@@ -328,6 +330,34 @@ def workload_e(E, O, digits):
     return out
 
 
+# ---- next to the path: decimal conversion (base_convert.h) ----------------
+
+def workload_convert(E, O):
+    """bigint_to_decimal of a random 10^6-digit integer (checked against the digit
+    string it was made from), and newton_reciprocal of a 2^22-bit integer."""
+    import random
+    import test_arith_ext as T
+    if hasattr(sys, "set_int_max_str_digits"):
+        sys.set_int_max_str_digits(0)
+    L = T._setup(E.lib())
+    t0 = time.perf_counter()
+    L.base_convert_init(20)
+    t_init = time.perf_counter() - t0
+    rng = random.Random(6)
+    s = "".join(rng.choice("0123456789") for _ in range(10 ** 6)).lstrip("0")
+    v = int(s)
+    t0 = time.perf_counter()
+    out = T.to_dec(L, v)
+    t_conv = time.perf_counter() - t0
+    q = rng.getrandbits(1 << 22) | (1 << ((1 << 22) - 1))
+    t0 = time.perf_counter()
+    x = T.recip(L, q)
+    t_rec = time.perf_counter() - t0
+    ok_rec = x * q <= 1 << (2 << 22) < (x + 1) * q
+    return {"digits": len(s), "to_decimal_s": t_conv, "exact": out == s, "cache_init_level20_s": t_init,
+            "reciprocal_2p22_bits_s": t_rec, "reciprocal_exact": bool(ok_rec)}
+
+
 # ---- config 5: serving -----------------------------------------------------
 
 def workload_serve(E, O):
@@ -417,7 +447,8 @@ def main():
         wl = {}
         todo = args.workloads.split(",")
         for name, fn in (("mul22", lambda: workload_mul22(E, O)), ("batch", lambda: workload_batch(E, O)),
-                         ("e", lambda: workload_e(E, O, args.e_digits)), ("serve", lambda: workload_serve(E, O))):
+                         ("e", lambda: workload_e(E, O, args.e_digits)), ("serve", lambda: workload_serve(E, O)),
+                         ("convert", lambda: workload_convert(E, O))):
             if name in todo:
                 progress(f"workload {name}")
                 wl[name] = fn()
