@@ -3,7 +3,9 @@
  *   - the schoolbook / NTT crossover (limbs per operand), timed on batches of
  *     equal-size products with each algorithm;
  *   - the pass split of every transform length (the largest sub-transform per
- *     pass, 2^8 .. 2^10), timed on one product per length.
+ *     pass, 2^8 .. 2^10), timed on one product per length;
+ *   - the tile shape of the long passes (tiles per block), timed on the
+ *     longest length.
  * Results can be saved to and loaded from a small JSON file.
  * (The reference fixes its crossover at 64 limbs, ntt/lib/arith/multiply.h:23-25,
  * and picks kernels by log_n, ntt/lib/arith/ntt_mul.hip:630-645.)
@@ -69,7 +71,7 @@ int ntt_engine_autotune_save(const char *path)
 {
     FILE *f = fopen(path, "w");
     if (!f) return fail(NTT_ERR_ARG, "cannot write %s", path);
-    fprintf(f, "{\"threshold\": %d, \"pass_cap\": {", g.threshold);
+    fprintf(f, "{\"threshold\": %d, \"tile_batch\": %d, \"pass_cap\": {", g.threshold, g.tile_batch);
     int first = 1;
     for (int l = 1; l <= NTT_MAX_LOG_LEN; l++)
         if (g.max_logs[l]) {
@@ -94,6 +96,10 @@ int ntt_engine_autotune_load(const char *path)
     const char *p = strstr(buf, "\"threshold\":");
     if (!p) return fail(NTT_ERR_ARG, "%s: no threshold", path);
     int thr = atoi(p + 12);
+    int tb = g.tile_batch;
+    const char *pt = strstr(buf, "\"tile_batch\":");
+    if (pt) tb = atoi(pt + 13);
+    if (tb < 1 || tb > 64) return fail(NTT_ERR_ARG, "%s: tile_batch %d out of range", path, tb);
     uint8_t caps[NTT_MAX_LOG_LEN + 1] = {};
     p = strstr(buf, "\"pass_cap\":");
     if (p) {
@@ -112,12 +118,14 @@ int ntt_engine_autotune_load(const char *path)
     }
     if (thr < 0 || thr > SB_MAX_OUT / 2) return fail(NTT_ERR_ARG, "%s: threshold %d out of range", path, thr);
     g.threshold = thr;
+    g.tile_batch = tb;
     memcpy(g.max_logs, caps, sizeof caps);
     return NTT_OK;
 }
 
-/* Measure and set the crossover and the pass caps for transform lengths
- * 2^min_log .. 2^max_log.  Takes a few seconds. */
+/* Measure and set the crossover, the pass caps for transform lengths
+ * 2^min_log .. 2^max_log and (max_log >= 24) the tile batch.  Takes a few
+ * seconds. */
 int ntt_engine_autotune(int min_log, int max_log)
 {
     std::lock_guard<std::recursive_mutex> lk(g_lock);
@@ -172,6 +180,32 @@ int ntt_engine_autotune(int min_log, int max_log)
             }
         }
         g.max_logs[l] = (uint8_t)best_cap;
+    }
+    /* 3. tile shape: tiles per block of the forward / inverse passes, timed on
+     * the longest transform.  A block takes more than one tile only when a
+     * pass has over 4096 blocks' worth of tiles (lengths of 2^24 and up). */
+    if (!rc && max_log >= 24) {
+        size_t n = std::min((size_t)1 << (max_log - 3), words / 4);
+        double best = 1e30;
+        const int saved_tb = g.tile_batch;
+        int best_tb = 1;
+        for (int tb : {1, 2, 4}) {
+            g.tile_batch = tb;
+            std::vector<double> t;
+            for (int r = 0; r < 4 && !rc; r++) {
+                double t0 = wall();
+                rc = enqueue_mul(buf + 2 * n, buf, n, buf + n, n, 0);
+                if (!rc) rc = sync_check();
+                if (r) t.push_back(wall() - t0);
+            }
+            if (rc) break;
+            std::sort(t.begin(), t.end());
+            if (t[t.size() / 2] < best) {
+                best = t[t.size() / 2];
+                best_tb = tb;
+            }
+        }
+        g.tile_batch = rc ? saved_tb : best_tb;
     }
     (void)hipFree(buf);
     return rc;

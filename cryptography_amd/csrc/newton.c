@@ -129,7 +129,7 @@ void bigint_div_newton(BigInt *q, const BigInt *N, const BigInt *D, const BigInt
         return;
     }
     const int b = bigint_bits(D), n = bigint_bits(N);
-    BigInt X = bigint_alloc(4), T = bigint_alloc(4), q0 = bigint_alloc(4);
+    BigInt X = bigint_alloc(4), q0 = bigint_alloc(4);
     int s;
     if (n <= 2 * b) {
         bigint_copy(&X, recip);
@@ -160,7 +160,6 @@ void bigint_div_newton(BigInt *q, const BigInt *N, const BigInt *D, const BigInt
     }
     bigint_copy(q, &q0);
     bigint_free(&X);
-    bigint_free(&T);
     bigint_free(&q0);
     bigint_free(&P);
     bigint_free(&r);

@@ -340,6 +340,11 @@ def set_tile_batch(tiles):
     return old
 
 
+def get_tile_batch():
+    """Current tiles per block of the long forward / inverse passes."""
+    return lib().ntt_engine_get_tile_batch()
+
+
 class ServeStats(ctypes.Structure):
     _fields_ = [("makespan_s", ctypes.c_double), ("batches", ctypes.c_size_t), ("max_inflight", ctypes.c_size_t)]
 

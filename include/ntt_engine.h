@@ -103,9 +103,11 @@ int ntt_engine_set_tile_batch(int tiles);
 int ntt_engine_get_tile_batch(void);
 
 /* Autotuner: times the schoolbook/NTT crossover on batches of equal-size
- * products and every pass split for lengths 2^min_log..2^max_log (11..28),
- * and sets the fastest.  A few seconds.  save/load keep the result in a
- * small JSON file ({"threshold": T, "pass_cap": {"<log_len>": cap, ...}}). */
+ * products, every pass split for lengths 2^min_log..2^max_log (11..28) and,
+ * when max_log >= 24, the tile batch (1, 2 or 4) on the longest length, and
+ * sets the fastest.  A few seconds.  save/load keep the result in a small
+ * JSON file ({"threshold": T, "tile_batch": B, "pass_cap": {"<log_len>": cap,
+ * ...}}); a file without "tile_batch" leaves the current value. */
 int ntt_engine_autotune(int min_log, int max_log);
 int ntt_engine_autotune_save(const char *path);
 int ntt_engine_autotune_load(const char *path);

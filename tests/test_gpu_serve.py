@@ -74,3 +74,21 @@ def test_autotune_keeps_results_exact(tmp_path):
         assert np.array_equal(E.mul(a, b), O.gmp_mul(a, b))
     thr2, caps2 = EN.autotune(11, 21, load=str(tmp_path / "tune.json"))
     assert thr2 == thr and caps2 == caps
+
+
+def test_autotune_tile_batch(tmp_path):
+    """Lengths of 2^24 and up also tune the tile batch; save/load keep it, and
+    products stay exact under the chosen value."""
+    saved = EN.get_tile_batch()
+    try:
+        EN.autotune(24, 24, save=str(tmp_path / "tune.json"))
+        tb = EN.get_tile_batch()
+        assert tb in (1, 2, 4)
+        a = O.splitmix(1 << 21, 7)
+        b = O.splitmix(1 << 21, 8)
+        assert np.array_equal(E.mul(a, b), O.gmp_mul(a, b))
+        EN.set_tile_batch(3)
+        EN.autotune(24, 24, load=str(tmp_path / "tune.json"))
+        assert EN.get_tile_batch() == tb
+    finally:
+        EN.set_tile_batch(saved)
