@@ -125,23 +125,34 @@ __host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
     y = reduce2<PI>(d);
 }
 
-/* Montgomery product a*b*2^-64 mod p for a, b < 2p; result in [0, 2p).
- * Every prime here is 1 mod 2^32, so -p^-1 = -1 (mod 2^32): the reduction
- * is two 32-bit steps whose multiplier m = -t mod 2^32 costs no multiply,
- * each adding m*p = m + m*p_hi*2^32 and dropping 32 bits.  The bound:
- * (a*b + m1*p + m2*p*2^32) / 2^64 < (4p/2^64) p + p + p/2^32 < 2p, as
- * 4p < 2^64 - 2^32 for these primes.  With a < 4p (an inverse-pass value)
- * and b < 2p the same sum stays below 2p + p + p/2^32 < 4p. */
+/* Montgomery product a*b*2^-64 mod p for a*b < 8p^2 (a < 4p and b < 2p);
+ * result in (0, 3p), and in (0, 2p) when a, b < 2p.
+ * Every prime here is 1 mod 2^32 (p = ph*2^32 + 1), so p^-1 = 1 (mod 2^32)
+ * and the reduction is two subtractive 32-bit steps with free multipliers:
+ * T = a*b = hi*2^64 + l1*2^32 + l0; T - l0*p = 2^32 (hi*2^32 + D) with
+ * D = l1 - l0*ph (signed); with s0 = D mod 2^32, (hi*2^32 + D) - s0*p =
+ * 2^32 (hi + floor(D / 2^32) - s0*ph).  That last value is
+ * (T - m p) / 2^64 for m = l0 + s0*2^32 < 2^64, so it lies in
+ * (T/2^64 - p, T/2^64]; adding p gives (T/2^64, T/2^64 + p], and
+ * T/2^64 < 8p^2/2^64 < 2p as p < 2^62.  The 128-bit product takes four
+ * 32x32->64 multiplies and each step one more: no carries to compare. */
 template <int PI>
 __host__ __device__ __forceinline__ uint64_t mont_mul_lazy(uint64_t a, uint64_t b)
 {
     static_assert((prime(PI) & 0xffffffffull) == 1, "p must be 1 mod 2^32");
-    constexpr uint64_t PH = prime(PI) >> 32;
-    const uint64_t lo = a * b, hi = mulhi64(a, b);
-    const uint32_t l0 = (uint32_t)lo, m1 = 0u - l0;
-    const uint64_t s = (lo >> 32) + (l0 != 0) + (uint64_t)m1 * PH;     /* (t + m1 p) / 2^32 - hi 2^32 */
-    const uint32_t s0 = (uint32_t)s, m2 = 0u - s0;
-    return hi + (s >> 32) + (s0 != 0) + (uint64_t)m2 * PH;
+    constexpr uint64_t P = prime(PI);
+    constexpr uint32_t NPH = 0u - (uint32_t)(P >> 32);             /* 2^32 - ph */
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t p00 = (uint64_t)a0 * b0;
+    const uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);
+    const uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;
+    const uint64_t hi = (uint64_t)a1 * b1 + (t >> 32) + (u >> 32);
+    const uint32_t l0 = (uint32_t)p00, l1 = (uint32_t)u;
+    const uint64_t dd = (uint64_t)l0 * NPH + l1;                   /* D + l0*2^32 */
+    const uint32_t s0 = (uint32_t)dd;                              /* D mod 2^32 */
+    const uint32_t dhi = (uint32_t)(dd >> 32) - l0;                /* floor(D / 2^32), two's complement */
+    const uint64_t x = hi + (uint64_t)(int64_t)(int32_t)dhi + P;
+    return (uint64_t)s0 * NPH + x - ((uint64_t)s0 << 32);          /* x - s0*ph */
 }
 
 /* Forward (decimation-in-frequency) butterfly on [0,2p) values:

@@ -57,6 +57,17 @@ __global__ void __launch_bounds__(256) k_small_twiddles(const TwPair *tw, TwPair
 #define LP(i) (i)
 #define FT_LDS (FT_SIZE)
 
+/* Block barrier for LDS traffic only: waits for this wave's LDS operations
+ * (lgkmcnt) but not for its global ones, so global stores, and LDS DMA
+ * transfers that a later explicit vmcnt wait covers, stay in flight across
+ * it (a release fence would wait for those too: a DMA transfer is an LDS
+ * write).  Threads of a pass block share data through LDS alone.  The
+ * memory clobber keeps the compiler from moving memory accesses across. */
+__device__ __forceinline__ void lds_sync()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 /* XCD-aware tile order: consecutive blocks go to different XCDs (round robin
  * over 8), so tile numbers are dealt out in 8 contiguous ranges, one per XCD;
  * neighbouring column tiles then share their 128-byte lines in one L2. */
@@ -162,7 +173,17 @@ struct XtwCtx {
     const TwPair *tw;     /* this prime's two-level table */
     uint32_t col0;        /* first column of the tile */
     uint32_t shift;       /* NTT_ROOT_LOG - log2(N): exponents of the 2^30 root */
+    uint64_t ta, tb;      /* A, B of this thread's group (Montgomery form) when
+                             the caller looked them up ahead (xt_ahead) */
 };
+
+/* When the cross-twiddle round has 8-point groups and one group per thread
+ * (S = 2^LOGS with LOGS a multiple of 3, so its groups are FT_SIZE/8 =
+ * FTPB), a thread's A and B are known at tile start: looking them up there
+ * overlaps the table loads with the tile transfer instead of stalling the
+ * round on them. */
+template <int LOGS>
+constexpr bool xt_ahead() { return LOGS % 3 == 0 && (FT_SIZE >> 3) == FTPB; }
 
 /* w^E * 2^64 mod p (Montgomery form), in [0, 2p), for E < 2^30. */
 template <int PI>
@@ -178,17 +199,41 @@ __device__ __forceinline__ uint64_t tw_mont(const TwPair *tw, uint32_t E)
 /* x[i] *= w^(EA + bitrev_RL(i) * EB) for i < 2^RL (exponents mod 2^30);
  * x in [0, 2p) gives [0, 2p), x in [0, 4p) (inverse passes) gives [0, 4p). */
 template <int PI, int RL>
-__device__ __forceinline__ void chain_twiddle(uint64_t *x, const TwPair *tw, uint32_t EA, uint32_t EB)
+__device__ __forceinline__ void chain_apply(uint64_t *x, uint64_t t, const uint64_t b)
 {
     constexpr int R = 1 << RL;
-    uint64_t t = tw_mont<PI>(tw, EA & ((1u << NTT_ROOT_LOG) - 1));
-    const uint64_t b = tw_mont<PI>(tw, EB & ((1u << NTT_ROOT_LOG) - 1));
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const int i = RL == 3 ? ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1) : RL == 2 ? ((k & 1) << 1) | (k >> 1) : k;
         x[i] = mont_mul_lazy<PI>(x[i], t);
         if (k + 1 < R) t = mont_mul_lazy<PI>(t, b);
     }
+}
+template <int PI, int RL>
+__device__ __forceinline__ void chain_twiddle(uint64_t *x, const TwPair *tw, uint32_t EA, uint32_t EB)
+{
+    chain_apply<PI, RL>(x, tw_mont<PI>(tw, EA & ((1u << NTT_ROOT_LOG) - 1)),
+                        tw_mont<PI>(tw, EB & ((1u << NTT_ROOT_LOG) - 1)));
+}
+
+/* A and B of this thread's group in the 8-point cross-twiddle round of tile
+ * column col0 (xt_ahead): the group is thread tid's, rows m0 .. m0+7 with
+ * m0 = ((tid >> logC) mod 2^(LOGS-3)) * 8, column r = col0 + (tid mod C);
+ * INV gives the inverse twiddles. */
+template <int PI, int LOGS, bool INV>
+__device__ __forceinline__ void xt_lookup(XtwCtx &xc, uint32_t logC)
+{
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const uint32_t r = xc.col0 + (tid & ((1u << logC) - 1));
+    const uint32_t m0 = ((tid >> logC) & ((1u << (LOGS - 3)) - 1)) << 3;
+    uint32_t ea = (r * brev_n(m0, LOGS)) << xc.shift, eb = (r << (LOGS - 3)) << xc.shift;
+    if (INV) {
+        ea = 0u - ea;
+        eb = 0u - eb;
+    }
+    xc.ta = tw_mont<PI>(xc.tw, ea & ((1u << NTT_ROOT_LOG) - 1));
+    xc.tb = tw_mont<PI>(xc.tw, eb & ((1u << NTT_ROOT_LOG) - 1));
 }
 
 /* ---- radix-R rounds over the LDS tile -------------------------------------
@@ -217,7 +262,9 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
         for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
         if constexpr (LD == 0) {
             const_dif<PI, RL, 0>(x);
-            if constexpr (XT) {
+            if constexpr (XT && xt_ahead<LOGS>()) {
+                chain_apply<PI, RL>(x, xc.ta, xc.tb);
+            } else if constexpr (XT) {
                 /* the outputs of rows m0 + i (m0 = blk << RL) of column col0 + c */
                 const uint32_t r = xc.col0 + c;
                 chain_twiddle<PI, RL>(x, xc.tw, (r * brev_n(m0, LOGS)) << xc.shift,
@@ -262,7 +309,9 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
 #pragma unroll
         for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
         if constexpr (LD == 0) {
-            if constexpr (XT) {
+            if constexpr (XT && xt_ahead<LOGS>()) {
+                chain_apply<PI, RL>(x, xc.ta, xc.tb);
+            } else if constexpr (XT) {
                 /* inputs of rows m0 + i of column col0 + c: inverse twiddles */
                 const uint32_t r = xc.col0 + c;
                 chain_twiddle<PI, RL>(x, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift),
@@ -296,7 +345,7 @@ __device__ __forceinline__ void dif_all(uint64_t *lds, const TwPair *twf, uint32
     if constexpr (LH >= 0) {
         constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
         dif_round<PI, LOGS, LH, RL, XT>(lds, twf, logC, xc);
-        __syncthreads();
+        lds_sync();
         dif_all<PI, LOGS, LH - RL, XT>(lds, twf, logC, xc);
     }
 }
@@ -309,7 +358,7 @@ __device__ __forceinline__ void dit_all(uint64_t *lds, const TwPair *twi, uint32
         constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
         dit_all<PI, LOGS, LH - RL, XT>(lds, twi, logC, xc);
         dit_round<PI, LOGS, LH, RL, XT>(lds, twi, logC, xc);
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -403,6 +452,36 @@ __device__ __forceinline__ void fill_first_tile(const PassArgs &pa, int op, cons
     }
 }
 
+/* Asynchronous copy of tile t of row x straight into LDS (no registers):
+ * each wave moves FT_SIZE/FTPB KB as 1 KB global_load_lds_dwordx4
+ * transfers, lane l carrying LDS words q, q+1 (q = chunk*128 + 2l) from
+ * their global places.  Those two are adjacent there too: both columns of
+ * one row when C >= 2, consecutive rows of the one column when T = 1 (C = 1
+ * otherwise never occurs; the launcher checks).  The caller waits (vmcnt)
+ * and syncs before reading the tile. */
+typedef __attribute__((address_space(3))) void lds_void_t;
+template <int LOGS>
+__device__ __forceinline__ void tile_dma(const PassArgs &pa, const uint64_t *x, const FGeom &g, uint32_t t,
+                                         uint64_t *lds)
+{
+    constexpr int CHUNKS = FT_SIZE / 128 / (FTPB / 64);    /* 1 KB chunks per wave */
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));     /* address math per call, not hoisted out of tile loops */
+    const uint32_t lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int j = 0; j < CHUNKS; j++) {
+        const uint32_t q0 = (w * CHUNKS + j) * 128;
+        uint32_t m, r;
+        const size_t a = res_at(ft_addr(LOGS, g, t, q0 + 2 * lane, m, r), pa.pad_log);
+        __builtin_amdgcn_global_load_lds((const void *)(x + a), (lds_void_t *)(lds + LP(q0)), 16, 0, 0);
+    }
+}
+/* transfers tile_dma leaves in flight per wave */
+#define TILE_DMA_OPS (FT_SIZE / 128 / (FTPB / 64))
+/* s_waitcnt vmcnt(N) (gfx9 encoding: vmcnt low bits [3:0], high bits [15:14];
+ * expcnt and lgkmcnt left at their maxima) */
+#define WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(0x0F70 | ((N) & 15) | (((N) >> 4) << 14))
+
 /* Tiles of a forward or inverse pass: block b takes tiles
  * [b*tpb, min((b+1)*tpb, ntiles)) of its (operand, prime) row, b in XCD
  * order, one after the other (the stage twiddles are staged once per block).
@@ -429,27 +508,27 @@ __device__ __forceinline__ void load_tile(const PassArgs &pa, const uint64_t *x,
 }
 
 template <int PI, int LOGS, bool FIRST>
-__device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds, const TwPair *twf)
+__device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds, TwPair *twf)
 {
     const FGeom g{pa.logT, pa.logC, pa.logG};
     uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
     const uint32_t colTiles = g.logT - g.logC;
-    XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT)};
+    XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0};
+    /* the first tile and the cross-twiddle look-ups are in flight while
+     * the stage twiddles are written */
+    stage_twiddles<PI, LOGS, 1>(pa.small_tw, twf, nullptr);
+    if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, t, lds);
+    xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+    if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, false>(xc, g.logC);
+    if constexpr (!FIRST) WAIT_VMCNT(0);
+    lds_sync();
     for (;;) {
         uint32_t tid;
         if constexpr (FIRST) {
             fill_first_tile<LOGS>(pa, op, g, t, lds);
-        } else {
-            uint64_t v[FT_PER];
-            load_tile<LOGS, false>(pa, x, op, g, t, v);
-            tid = threadIdx.x;
-            asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
-#pragma unroll
-            for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
+            lds_sync();
         }
-        __syncthreads();
-        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
         dif_all<PI, LOGS, LOGS - 1, true>(lds, twf, g.logC, xc);   /* cross twiddles in the last round */
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
@@ -460,7 +539,14 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
             for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
         }
         if (++t >= tend) break;
-        __syncthreads();
+        lds_sync();
+        if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, t, lds);
+        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+        if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, false>(xc, g.logC);
+        if constexpr (!FIRST) {
+            WAIT_VMCNT(0);
+            lds_sync();
+        }
     }
 }
 
@@ -471,26 +557,35 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
  * rounds carry no twiddle code (fewer live registers).  Otherwise they are
  * applied in the first DIT round. */
 template <int PI, int LOGS, bool LOADCHAIN>
-__device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, const TwPair *twi)
+__device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair *twi)
 {
     const FGeom g{pa.logT, pa.logC, pa.logG};
     uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
     const uint32_t colTiles = g.logT - g.logC;
-    XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT)};
+    XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0};
+    /* without the load chain the tile goes straight into LDS, in flight
+     * while the stage twiddles are written */
+    stage_twiddles<PI, LOGS, 2>(pa.small_tw, nullptr, twi);
+    xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+    if constexpr (!LOADCHAIN) {
+        tile_dma<LOGS>(pa, x, g, t, lds);
+        if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, true>(xc, g.logC);
+        WAIT_VMCNT(0);
+    }
+    lds_sync();
     for (;;) {
-        uint64_t v[FT_PER];
-        load_tile<LOGS, false>(pa, x, 0, g, t, v);
         uint32_t tid = threadIdx.x;
-        asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
-        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
         if constexpr (LOADCHAIN) {
+            uint64_t v[FT_PER];
+            load_tile<LOGS, false>(pa, x, 0, g, t, v);
+            asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
             const uint32_t r = xc.col0 + (tid & ((1u << g.logC) - 1)), m0 = tid >> g.logC;
             chain_twiddle<PI, 3>(v, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift), 0u - (r << xc.shift));
-        }
 #pragma unroll
-        for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
-        __syncthreads();
+            for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
+            lds_sync();
+        }
         /* otherwise the inverse cross twiddles go into the first round */
         dit_all<PI, LOGS, LOGS - 1, !LOADCHAIN>(lds, twi, g.logC, xc);
         tid = threadIdx.x;
@@ -502,7 +597,14 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, const 
             for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
         }
         if (++t >= tend) break;
-        __syncthreads();
+        lds_sync();
+        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+        if constexpr (!LOADCHAIN) {
+            tile_dma<LOGS>(pa, x, g, t, lds);
+            if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, true>(xc, g.logC);
+            WAIT_VMCNT(0);
+            lds_sync();
+        }
     }
 }
 
@@ -514,7 +616,7 @@ __device__ __forceinline__ void dif_upper(uint64_t *lds, const TwPair *twf)
     constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
     if constexpr (LH - RL >= 0) {
         dif_round<PI, LOGS, LH, RL, false>(lds, twf, 0, XtwCtx{});
-        __syncthreads();
+        lds_sync();
         dif_upper<PI, LOGS, LH - RL>(lds, twf);
     }
 }
@@ -525,7 +627,7 @@ __device__ __forceinline__ void dit_upper(uint64_t *lds, const TwPair *twi)
     if constexpr (LH - RL >= 0) {
         dit_upper<PI, LOGS, LH - RL>(lds, twi);
         dit_round<PI, LOGS, LH, RL, false>(lds, twi, 0, XtwCtx{});
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -548,13 +650,20 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     uint64_t ra[FT_PER];
     size_t base = 0, step = 0;
-    if constexpr (!FIRST) tile_span<LOGS>(pa, g, t, threadIdx.x, base, step);
+    if constexpr (FIRST) {
+        stage_twiddles<PI, LOGS, 3>(pa.small_tw, twf, twi);
 #pragma unroll
-    for (int k = 0; k < FT_PER; k++) {
-        uint32_t q = threadIdx.x + k * FTPB, m, r;
-        lds[LP(q)] = FIRST ? first_in(pa, 0, ft_addr(LOGS, g, t, q, m, r)) : xa[base + k * step];
+        for (int k = 0; k < FT_PER; k++) {
+            uint32_t q = threadIdx.x + k * FTPB, m, r;
+            lds[LP(q)] = first_in(pa, 0, ft_addr(LOGS, g, t, q, m, r));
+        }
+    } else {
+        /* a's tile is in flight while the stage twiddles are written */
+        tile_dma<LOGS>(pa, xa, g, t, lds);
+        stage_twiddles<PI, LOGS, 3>(pa.small_tw, twf, twi);
+        WAIT_VMCNT(0);
     }
-    __syncthreads();
+    lds_sync();
     dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
     /* a's smallest round, kept in registers */
 #pragma unroll
@@ -565,14 +674,22 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
         for (int i = 0; i < RR; i++) ra[n * RR + i] = lds[LP(base + i)];
         const_dif<PI, RLL, 0>(ra + n * RR);
     }
+    /* b's tile into the same LDS once a's is in registers: a second buffer
+     * would let it arrive during a's rounds, but at 80 KB per block two
+     * blocks share a CU instead of three, which was measured slower */
     if (!pa.square) {
-        __syncthreads();
+        lds_sync();
+        if constexpr (FIRST) {
 #pragma unroll
-        for (int k = 0; k < FT_PER; k++) {
-            uint32_t q = threadIdx.x + k * FTPB, m, r;
-            lds[LP(q)] = FIRST ? first_in(pa, 1, ft_addr(LOGS, g, t, q, m, r)) : xb[base + k * step];
+            for (int k = 0; k < FT_PER; k++) {
+                uint32_t q = threadIdx.x + k * FTPB, m, r;
+                lds[LP(q)] = first_in(pa, 1, ft_addr(LOGS, g, t, q, m, r));
+            }
+        } else {
+            tile_dma<LOGS>(pa, xb, g, t, lds);
+            WAIT_VMCNT(0);
         }
-        __syncthreads();
+        lds_sync();
         dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
     }
     const uint64_t s = pa.scale[PI], sp = pa.scale_p[PI];
@@ -595,9 +712,9 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 #pragma unroll
         for (int i = 0; i < RR; i++) lds[LP(base + i)] = x[i];
     }
-    __syncthreads();
+    lds_sync();
     dit_upper<PI, LOGS, LOGS - 1>(lds, twi);
-    if constexpr (FIRST) tile_span<LOGS>(pa, g, t, threadIdx.x, base, step);
+    tile_span<LOGS>(pa, g, t, threadIdx.x, base, step);
 #pragma unroll
     for (int k = 0; k < FT_PER; k++) xa[base + k * step] = lds[LP(threadIdx.x + k * FTPB)];
 }
@@ -611,9 +728,8 @@ template <int PI, int LOGS, int KIND>
 __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, int op,
                                           uint64_t *lds, TwPair *twf, TwPair *twi)
 {
-    constexpr int DIRS = (KIND == 0 || KIND == 3) ? 1 : (KIND == 1 || KIND == 5) ? 2 : 3;
-    stage_twiddles<PI, LOGS, DIRS>(pa.small_tw, twf, twi);
-    __syncthreads();
+    /* each routine stages its stage twiddles itself, after starting its
+     * first tile's transfer */
     if constexpr (KIND == 0) fwd_tiles<PI, LOGS, false>(pa, xa, op, lds, twf);
     else if constexpr (KIND == 3) fwd_tiles<PI, LOGS, true>(pa, xa, op, lds, twf);
     else if constexpr (KIND == 1) inv_tiles<PI, LOGS, false>(pa, xa, lds, twi);
@@ -629,6 +745,7 @@ __global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
     constexpr int NTW = (KIND == 2 || KIND == 4) ? 2 : 1;
     __shared__ uint64_t lds[FT_LDS];
     __shared__ TwPair tws[NTW << LOGS];
+
     static_assert((1 << LOGS) <= TW_LDS_MAX, "stage twiddles exceed LDS table");
     TwPair *twf = tws, *twi = tws + (NTW - 1) * (1 << LOGS);
     const uint32_t pi = blockIdx.y, op = blockIdx.z;
@@ -647,6 +764,8 @@ hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntile
     const bool looped = kind == 0 || kind == 1 || kind == 3 || kind == 5;
     if (looped && (pa.tpb < 1 || pa.ntiles != ntiles)) return hipErrorInvalidValue;
     if (kind == 5 && (logS < 3 || pa.logC + logS != 12)) return hipErrorInvalidValue;
+    /* tile transfers move column pairs (C >= 2) or row pairs (T = 1) */
+    if ((kind == 0 || kind == 1) && pa.logC == 0 && pa.logT != 0) return hipErrorInvalidValue;
     const size_t nblk = looped ? (ntiles + pa.tpb - 1) / pa.tpb : ntiles;
     dim3 grid((unsigned)nblk, NTT_NPRIMES, (kind == 0 || kind == 3) ? nops : 1);
     dim3 block(FTPB);
