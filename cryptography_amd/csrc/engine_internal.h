@@ -8,7 +8,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/ntt_engine.h"
@@ -77,12 +80,37 @@ struct PassPlan {
     uint32_t logS[8];
 };
 
+/* Allocator whose resize() leaves trivially constructible elements
+ * uninitialized: descriptor arrays of millions of entries are written in
+ * full right after (in parallel), so zero-filling them first on one thread
+ * only costs time. */
+template <class T>
+struct uninit_alloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = uninit_alloc<U>;
+    };
+    uninit_alloc() = default;
+    template <class U>
+    uninit_alloc(const uninit_alloc<U> &) noexcept {}
+    template <class U>
+    void construct(U *p) noexcept
+    {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... Args>
+    void construct(U *p, Args &&...args)
+    {
+        ::new ((void *)p) U(std::forward<Args>(args)...);
+    }
+};
+
 struct BatchPlan {
     struct Class {
         int logL;            /* 0: schoolbook */
         uint32_t first, count, max_nout;
     };
-    std::vector<MulDesc> sorted;     /* descriptors grouped by class */
+    std::vector<MulDesc, uninit_alloc<MulDesc>> sorted;   /* descriptors grouped by class */
     std::vector<Class> classes;
     size_t res_words = 0, scratch = 0;
 };
@@ -111,6 +139,30 @@ void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
                       uint32_t max_nout, int square, hipStream_t st, int residues_in = 0);
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);
+
+/* Host threads for planning work (descriptor sorting, tree construction):
+ * at most 16, the share of a GPU's host cores this library assumes. */
+inline unsigned host_threads()
+{
+    const unsigned hc = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(16u, hc ? hc : 1u));
+}
+/* f(begin, end, worker) over contiguous chunks of [0, n), one per thread
+ * (inline for n below two chunks of min_chunk).  Returns the chunk count. */
+template <class F>
+unsigned par_chunks(size_t n, F &&f, size_t min_chunk = (size_t)1 << 15)
+{
+    const unsigned nt = (unsigned)std::max<size_t>(1, std::min<size_t>(host_threads(), n / min_chunk));
+    if (nt <= 1) {
+        f((size_t)0, n, 0u);
+        return 1;
+    }
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (unsigned w = 0; w < nt; w++) th.emplace_back([&f, n, nt, w] { f(n * w / nt, n * (w + 1) / nt, w); });
+    for (auto &x : th) x.join();
+    return nt;
+}
 int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hipStream_t st);
 int sync_check();
 int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int square);

@@ -411,25 +411,77 @@ int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
      * within a class and costs O(n) */
     constexpr int NK = NTT_MAX_LOG_LEN + 2;
     std::vector<uint8_t> key(n);
-    size_t cnt[NK + 1] = {};
-    uint32_t maxo[NK] = {};
-    for (size_t i = 0; i < n; i++) {
-        MulDesc d;
-        memcpy(&d, &h[i], sizeof d);
-        int rc = check_desc(d);
-        if (rc) return rc;
-        int k = (d.na == 0 || d.nb == 0) ? -1 : use_schoolbook(d.na, d.nb, d.n_out) ? 0 : log_len(d.na, d.nb);
-        key[i] = (uint8_t)(k + 1);
-        cnt[k + 2]++;
-        maxo[k + 1] = std::max(maxo[k + 1], std::max(d.n_out, d.na + d.nb));
+    /* large batches (the product tree's lower levels) are keyed and
+     * scattered by chunks in parallel; the order within a class is the
+     * batch order either way */
+    const unsigned nch = std::max<unsigned>(1, std::min<unsigned>(host_threads(), (unsigned)(n >> 15)));
+    std::vector<size_t> ccnt((size_t)nch * NK, 0);
+    std::vector<uint32_t> cmax((size_t)nch * NK, 0);
+    std::vector<size_t> cbad(nch, SIZE_MAX);
+    auto classify = [&](size_t lo, size_t hi, unsigned w) {
+        size_t *cnt = &ccnt[(size_t)w * NK];
+        uint32_t *mx = &cmax[(size_t)w * NK];
+        for (size_t i = lo; i < hi; i++) {
+            MulDesc d;
+            memcpy(&d, &h[i], sizeof d);
+            if ((d.na && !d.a) || (d.nb && !d.b) || (d.n_out && !d.c) || (d.nadd && !d.add) || d.n_out < d.nadd ||
+                (d.n_out == 0 && d.na && d.nb) || log_len(d.na, d.nb) > NTT_MAX_LOG_LEN) {
+                cbad[w] = i;
+                return;
+            }
+            int k = (d.na == 0 || d.nb == 0) ? -1 : use_schoolbook(d.na, d.nb, d.n_out) ? 0 : log_len(d.na, d.nb);
+            key[i] = (uint8_t)(k + 1);
+            cnt[k + 1]++;
+            mx[k + 1] = std::max(mx[k + 1], std::max(d.n_out, d.na + d.nb));
+        }
+    };
+    if (nch > 1) {
+        std::vector<std::thread> th;
+        for (unsigned w = 0; w < nch; w++) th.emplace_back(classify, n * w / nch, n * (w + 1) / nch, w);
+        for (auto &x : th) x.join();
+    } else {
+        classify(0, n, 0);
     }
-    for (int k = 0; k < NK; k++) cnt[k + 1] += cnt[k];   /* cnt[k] = first slot of key k */
+    for (unsigned w = 0; w < nch; w++)
+        if (cbad[w] != SIZE_MAX) {
+            MulDesc d;
+            memcpy(&d, &h[cbad[w]], sizeof d);
+            return check_desc(d);   /* the same test, with its message */
+        }
+    /* first slot of key k for chunk w: all chunks' keys below k, then the
+     * earlier chunks' key k */
+    size_t total[NK] = {};
+    uint32_t maxo[NK] = {};
+    for (unsigned w = 0; w < nch; w++)
+        for (int k = 0; k < NK; k++) {
+            total[k] += ccnt[(size_t)w * NK + k];
+            maxo[k] = std::max(maxo[k], cmax[(size_t)w * NK + k]);
+        }
+    size_t first[NK + 1] = {};
+    for (int k = 0; k < NK; k++) first[k + 1] = first[k] + total[k];
+    std::vector<size_t> slot((size_t)nch * NK);
+    for (int k = 0; k < NK; k++) {
+        size_t at = first[k];
+        for (unsigned w = 0; w < nch; w++) {
+            slot[(size_t)w * NK + k] = at;
+            at += ccnt[(size_t)w * NK + k];
+        }
+    }
     bp.sorted.resize(n);
     bp.classes.clear();
     for (int k = 0; k < NK; k++)
-        if (cnt[k + 1] > cnt[k])
-            bp.classes.push_back({k - 1, (uint32_t)cnt[k], (uint32_t)(cnt[k + 1] - cnt[k]), maxo[k]});
-    for (size_t i = 0; i < n; i++) memcpy(&bp.sorted[cnt[key[i]]++], &h[i], sizeof(MulDesc));
+        if (total[k]) bp.classes.push_back({k - 1, (uint32_t)first[k], (uint32_t)total[k], maxo[k]});
+    auto scatter = [&](size_t lo, size_t hi, unsigned w) {
+        size_t *sl = &slot[(size_t)w * NK];
+        for (size_t i = lo; i < hi; i++) memcpy(&bp.sorted[sl[key[i]]++], &h[i], sizeof(MulDesc));
+    };
+    if (nch > 1) {
+        std::vector<std::thread> th;
+        for (unsigned w = 0; w < nch; w++) th.emplace_back(scatter, n * w / nch, n * (w + 1) / nch, w);
+        for (auto &x : th) x.join();
+    } else {
+        scatter(0, n, 0);
+    }
     bp.res_words = bp.scratch = 0;
     for (auto &c : bp.classes) {
         if (c.logL <= 0) continue;

@@ -78,8 +78,6 @@ __global__ void __launch_bounds__(256) k_leaves(const LeafDesc *leaves, uint32_t
 
 struct TreeNode {
     long a, b;
-    int left = -1, right = -1;  /* child node indices; -1 for a leaf */
-    int height = 0;
     uint64_t offP = 0, offB = 0;
     uint32_t capP = 0, capB = 0;
 };
@@ -92,27 +90,18 @@ static uint32_t cap_limbs_B(long a, long b)
     return (uint32_t)(bits / 64.0 + 1e-9 * bits) + 2;
 }
 
-static int build_nodes(long a, long b, std::vector<TreeNode> &nodes)
+/* Height of a node with s terms: 0 for a leaf, else one more than its larger
+ * child's (the right one, ceil(s/2) terms). */
+static int node_height(long s)
 {
-    TreeNode nd;
-    nd.a = a;
-    nd.b = b;
-    int idx = (int)nodes.size();
-    nodes.push_back(nd);
-    if (b - a > LEAF_TERMS) {
-        long m = a + (b - a) / 2;
-        int l = build_nodes(a, m, nodes);
-        int r = build_nodes(m, b, nodes);
-        nodes[idx].left = l;
-        nodes[idx].right = r;
-        nodes[idx].height = 1 + std::max(nodes[l].height, nodes[r].height);
-    }
-    return idx;
+    int h = 0;
+    for (; s > LEAF_TERMS; s -= s / 2) h++;
+    return h;
 }
 
 struct ntt_split_tree {
     long a = 0, b = 0;
-    std::vector<TreeNode> nodes;
+    TreeNode root;
     uint64_t *arena = nullptr;
     size_t arena_words = 0;
     LeafDesc *d_leaves = nullptr;
@@ -186,59 +175,114 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
     t->a = a;
     t->b = b;
     t->per_product = use_graph == 2;
-    if (b <= a) {
-        TreeNode nd;
-        nd.a = a;
-        nd.b = b;
-        t->nodes.push_back(nd);
-    } else {
-        t->nodes.reserve(2 * (size_t)((b - a + LEAF_TERMS - 1) / LEAF_TERMS) + 2);
-        build_nodes(a, b, t->nodes);
-    }
-    /* capacities and arena offsets */
-    size_t off = 0;
-    int H = 0;
-    for (auto &nd : t->nodes) {
-        if (nd.left < 0) {
-            /* leaf: B < prod 2^bitlen(k) over at most LEAF_TERMS terms, P < 2B,
-             * both within LEAF_LIMBS */
-            uint32_t bits = 0;
-            for (long k = nd.a; k < nd.b; k++) bits += 64 - __builtin_clzl((unsigned long)k);
-            nd.capB = std::max<uint32_t>(std::min<uint32_t>(bits / 64 + 1, LEAF_LIMBS), 1);
-            nd.capP = std::max<uint32_t>(std::min<uint32_t>((bits + 1) / 64 + 1, LEAF_LIMBS), 1);
-        } else {
-            nd.capB = cap_limbs_B(nd.a, nd.b);
-            nd.capP = nd.capB + 1;
+    /* The tree in one array, depth by depth, left to right: node i covers
+     * [lo[i], lo[i] + sz[i]); an internal node's children are kid[i] and
+     * kid[i] + 1.  Then three passes over all nodes in parallel chunks:
+     * (A) heights, capacities, and per chunk its arena words, leaves and
+     * products per height; (B) arena offsets and the leaf descriptors;
+     * (C) the product descriptors, which read the children's offsets. */
+    std::vector<long> lo{a};
+    std::vector<uint32_t> sz{(uint32_t)std::max<long>(b - a, 0)};
+    std::vector<uint32_t> kid{0};
+    for (size_t d0 = 0, d1 = 1; d0 < d1; d0 = d1, d1 = lo.size()) {
+        for (size_t i = d0; i < d1; i++) {
+            const uint32_t s_ = sz[i];
+            if (s_ <= LEAF_TERMS) continue;
+            kid[i] = (uint32_t)lo.size();
+            lo.push_back(lo[i]);                   /* [a, m), m = a + (b-a)/2 */
+            sz.push_back(s_ / 2);
+            lo.push_back(lo[i] + s_ / 2);
+            sz.push_back(s_ - s_ / 2);
+            kid.push_back(0);
+            kid.push_back(0);
         }
-        nd.offP = off;
-        off += nd.capP;
-        nd.offB = off;
-        off += nd.capB;
-        H = std::max(H, nd.height);
     }
+    const size_t N = lo.size();
+    std::vector<uint8_t, uninit_alloc<uint8_t>> height(N);
+    std::vector<uint32_t, uninit_alloc<uint32_t>> capP(N), capB(N);
+    std::vector<uint64_t, uninit_alloc<uint64_t>> offP(N), offB(N);
+    const unsigned NT = host_threads();
+    constexpr int HMAX = 40;                       /* heights < 33 for 2^32 terms */
+    std::vector<uint64_t> c_words(NT, 0), c_leaves(NT, 0);
+    std::vector<uint64_t> c_prod((size_t)NT * (HMAX + 1), 0);
+    const unsigned nch = par_chunks(N, [&](size_t i0, size_t i1, unsigned w) {
+        uint64_t words = 0, nl = 0;
+        uint64_t *np = &c_prod[(size_t)w * (HMAX + 1)];
+        for (size_t i = i0; i < i1; i++) {
+            const long l = lo[i], r = l + sz[i];
+            const int h = node_height(sz[i]);
+            height[i] = (uint8_t)h;
+            if (h == 0) {
+                /* leaf: B < prod 2^bitlen(k) over at most LEAF_TERMS terms,
+                 * P < 2B, both within LEAF_LIMBS */
+                uint32_t bits = 0;
+                for (long k = l; k < r; k++) bits += 64 - __builtin_clzl((unsigned long)k);
+                capB[i] = std::max<uint32_t>(std::min<uint32_t>(bits / 64 + 1, LEAF_LIMBS), 1);
+                capP[i] = std::max<uint32_t>(std::min<uint32_t>((bits + 1) / 64 + 1, LEAF_LIMBS), 1);
+                nl++;
+            } else {
+                capB[i] = cap_limbs_B(l, r);
+                capP[i] = capB[i] + 1;
+                np[h] += 2;
+            }
+            words += capP[i] + capB[i];
+        }
+        c_words[w] = words;
+        c_leaves[w] = nl;
+    });
+    /* chunk starts: arena words and leaves in node order; products per
+     * height, chunk after chunk */
+    int H = 0;
+    uint64_t off = 0, nleaf = 0;
+    std::vector<size_t> per_n(HMAX + 1, 0);
+    for (unsigned w = 0; w < nch; w++) {
+        const uint64_t ww = c_words[w], lw = c_leaves[w];
+        c_words[w] = off;
+        off += ww;
+        c_leaves[w] = nleaf;
+        nleaf += lw;
+        for (int h = 1; h <= HMAX; h++) {
+            const uint64_t v = c_prod[(size_t)w * (HMAX + 1) + h];
+            c_prod[(size_t)w * (HMAX + 1) + h] = per_n[h];
+            per_n[h] += v;
+            if (v) H = std::max(H, h);
+        }
+    }
+    std::vector<LeafDesc, uninit_alloc<LeafDesc>> leaves(nleaf);
+    par_chunks(N, [&](size_t i0, size_t i1, unsigned w) {
+        uint64_t o = c_words[w], li = c_leaves[w];
+        for (size_t i = i0; i < i1; i++) {
+            offP[i] = o;
+            o += capP[i];
+            offB[i] = o;
+            o += capB[i];
+            if (height[i] == 0)
+                leaves[li++] = {(uint64_t)lo[i], (uint64_t)(lo[i] + sz[i]), offP[i], offB[i], capP[i], capB[i]};
+        }
+    });
+    t->root = TreeNode{a, b, offP[0], offB[0], capP[0], capB[0]};
     t->arena_words = off;
     int rc = NTT_OK;
     if (hipMalloc((void **)&t->arena, off * 8) != hipSuccess) rc = fail(NTT_ERR_NOMEM, "tree arena (%zu limbs)", off);
-    /* leaves */
-    std::vector<LeafDesc> leaves;
-    for (auto &nd : t->nodes)
-        if (nd.left < 0)
-            leaves.push_back({(uint64_t)nd.a, (uint64_t)nd.b, nd.offP, nd.offB, nd.capP, nd.capB});
     t->n_leaves = (uint32_t)leaves.size();
     if (!rc && hipMalloc((void **)&t->d_leaves, leaves.size() * sizeof(LeafDesc)) != hipSuccess)
         rc = fail(NTT_ERR_NOMEM, "leaf descriptors");
     if (!rc) rc = copy_h2d(t->d_leaves, leaves.data(), leaves.size() * sizeof(LeafDesc));
-    /* one batch per height: B = B_L*B_R and P = P_L*B_R + P_R */
-    std::vector<std::vector<ntt_mul_desc>> per(H + 1);
-    for (auto &nd : t->nodes) {
-        if (nd.left < 0) continue;
-        const TreeNode &L = t->nodes[nd.left], &R = t->nodes[nd.right];
-        uint64_t *A = t->arena;
-        ntt_mul_desc dB{A + L.offB, A + R.offB, A + nd.offB, nullptr, L.capB, R.capB, 0, nd.capB};
-        ntt_mul_desc dP{A + L.offP, A + R.offB, A + nd.offP, A + R.offP, L.capP, R.capB, R.capP, nd.capP};
-        per[nd.height].push_back(dB);
-        per[nd.height].push_back(dP);
-    }
+    /* one batch per height: B = B_L*B_R and P = P_L*B_R + P_R, in node order */
+    std::vector<std::vector<ntt_mul_desc, uninit_alloc<ntt_mul_desc>>> per(H + 1);
+    for (int h = 1; h <= H; h++) per[h].resize(per_n[h]);
+    uint64_t *A = t->arena;
+    par_chunks(N, [&](size_t i0, size_t i1, unsigned w) {
+        uint64_t *at = &c_prod[(size_t)w * (HMAX + 1)];
+        for (size_t i = i0; i < i1; i++) {
+            const int h = height[i];
+            if (!h) continue;
+            const size_t l = kid[i], r = l + 1;
+            per[h][at[h]++] = ntt_mul_desc{A + offB[l], A + offB[r], A + offB[i], nullptr, capB[l], capB[r], 0, capB[i]};
+            per[h][at[h]++] = ntt_mul_desc{A + offP[l], A + offB[r], A + offP[i], A + offP[r], capP[l], capB[r], capP[r],
+                                           capP[i]};
+        }
+    });
     t->levels.resize(H + 1);
     t->level_off.resize(H + 1);
     size_t ndesc = 0, res_words = 1, scratch = 64;
@@ -314,7 +358,7 @@ int ntt_split_tree_device_result(ntt_split_tree *t, const uint64_t **dP, size_t 
                                  size_t *capB)
 {
     if (!t) return fail(NTT_ERR_ARG, "null tree");
-    const TreeNode &r = t->nodes[0];
+    const TreeNode &r = t->root;
     if (dP) *dP = t->arena + r.offP;
     if (capP) *capP = r.capP;
     if (dB) *dB = t->arena + r.offB;
@@ -328,7 +372,7 @@ int ntt_split_tree_result(ntt_split_tree *t, uint64_t *P, size_t capP, size_t *n
 {
     std::lock_guard<std::recursive_mutex> lk(g_lock);
     if (!t) return fail(NTT_ERR_ARG, "null tree");
-    const TreeNode &r = t->nodes[0];
+    const TreeNode &r = t->root;
     if (capP < r.capP || capB < r.capB)
         return fail(NTT_ERR_ARG, "result buffers too small (need %u and %u limbs)", r.capP, r.capB);
     int rc = copy_d2h(P, t->arena + r.offP, (size_t)r.capP * 8);
@@ -345,8 +389,8 @@ int ntt_split_tree_result(ntt_split_tree *t, uint64_t *P, size_t capP, size_t *n
 int ntt_split_tree_root_caps(ntt_split_tree *t, size_t *capP, size_t *capB)
 {
     if (!t) return fail(NTT_ERR_ARG, "null tree");
-    if (capP) *capP = t->nodes[0].capP;
-    if (capB) *capB = t->nodes[0].capB;
+    if (capP) *capP = t->root.capP;
+    if (capB) *capB = t->root.capB;
     return NTT_OK;
 }
 
