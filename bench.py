@@ -334,7 +334,8 @@ def workload_e(E, O, digits):
 
 def workload_convert(E, O):
     """bigint_to_decimal of a random 10^6-digit integer (checked against the digit
-    string it was made from), and newton_reciprocal of a 2^22-bit integer."""
+    string it was made from), newton_reciprocal of a 2^22-bit integer, and
+    ntt_to_decimal of a 10^7-digit integer against GMP (timed on one host core)."""
     import random
     import test_arith_ext as T
     if hasattr(sys, "set_int_max_str_digits"):
@@ -354,8 +355,28 @@ def workload_convert(E, O):
     x = T.recip(L, q)
     t_rec = time.perf_counter() - t0
     ok_rec = x * q <= 1 << (2 << 22) < (x + 1) * q
+    # 10^7 digits straight from limbs (ntt_to_decimal), checked against GMP's mpz_get_str
+    import ctypes
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    L.ntt_to_decimal.argtypes = [u64p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t,
+                                 ctypes.POINTER(ctypes.c_size_t)]
+    limbs = O.splitmix(7, 520000)
+    cap = int(64 * len(limbs) * 0.30103) + 2
+    buf, nd = ctypes.create_string_buffer(cap), ctypes.c_size_t()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        rc = L.ntt_to_decimal(limbs.ctypes.data_as(u64p), len(limbs), buf, cap, ctypes.byref(nd))
+        ts.append(time.perf_counter() - t0)
+        if rc:
+            raise RuntimeError(f"ntt_to_decimal failed ({rc})")
+    t0 = time.perf_counter()
+    ref = O.gmp_to_decimal(limbs)
+    t_gmp = time.perf_counter() - t0
     return {"digits": len(s), "to_decimal_s": t_conv, "exact": out == s, "cache_init_level20_s": t_init,
-            "reciprocal_2p22_bits_s": t_rec, "reciprocal_exact": bool(ok_rec)}
+            "reciprocal_2p22_bits_s": t_rec, "reciprocal_exact": bool(ok_rec),
+            "to_decimal_1e7": {"digits": nd.value, "s": float(np.median(ts)), "exact": buf.raw[:nd.value].decode() == ref,
+                               "cpu_gmp_s": t_gmp}}
 
 
 # ---- config 5: serving -----------------------------------------------------

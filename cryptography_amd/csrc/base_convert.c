@@ -10,6 +10,9 @@
  * division by 10^19 (the splits there would be GPU launches of a few limbs
  * each); values below 10^16 are printed directly.  The reference's
  * algorithm is the same divide and conquer (base_convert.c:135-259).
+ * Above DECIMAL_DEVICE_LIMBS limbs the whole conversion runs on the device
+ * instead (ntt_to_decimal: a fixed-point radix tree with one batched
+ * multiply per level), with the same result.
  */
 #include <stdint.h>
 #include <stdio.h>
@@ -20,10 +23,12 @@
 #include "../../include/base_convert.h"
 #include "../../include/multiply.h"
 #include "../../include/newton.h"
+#include "../../include/ntt_engine.h"
 
 #define MAX_LEVEL 40
 #define DIRECT_LEVEL 3        /* below 10^(2^(3+1)) = 10^16: one u64 */
 #define BASE_LEVEL 10         /* below 10^(2^11): converted on the host by repeated division by 10^19 */
+#define DECIMAL_DEVICE_LIMBS 256   /* above this (about 4900 digits): ntt_to_decimal, all on the device */
 
 static BigInt pow10_k[MAX_LEVEL + 1], recip_k[MAX_LEVEL + 1];
 static int cache_level = -1;
@@ -185,6 +190,19 @@ char *bigint_to_decimal(const BigInt *N)
         bigint_free(&sq);
         if (!fits) return NULL;
         K = cache_level;
+    }
+    if (N->n_limbs > DECIMAL_DEVICE_LIMBS) {
+        /* one device-resident conversion instead of a division per node */
+        if (!ntt_engine_is_init() && ntt_engine_init(0)) die("engine initialisation failed");
+        size_t cap = (size_t)((double)bigint_bits(N) * 0.30103) + 2, len = 0;
+        char *s = (char *)malloc(cap);
+        if (!s) die("out of memory");
+        int rc = ntt_to_decimal(N->limbs, (size_t)N->n_limbs, s, cap, &len);
+        if (rc) {
+            fprintf(stderr, "base_convert: %s\n", ntt_engine_last_error());
+            die("device decimal conversion failed");
+        }
+        return s;
     }
     if (K < DIRECT_LEVEL) K = DIRECT_LEVEL;   /* the direct case covers everything below 10^16 */
     size_t cap = ((size_t)1 << (K + 1)) + 1;

@@ -182,6 +182,43 @@ __global__ void __launch_bounds__(256) k_leaf_digits(const DigitLeaf *leaves, si
     }
 }
 
+/* Exact digits across the radix tree's truncations.  A child's fraction is
+ * its node's, truncated to the child's precision, so it lies at most one unit
+ * of its last limb below the exact value; its digits are still exact unless
+ * the value beyond its digit range (its tail, in units of its last digit) is
+ * smaller than that error, where the truncation would borrow from the last
+ * digit.  The left child's tail is its right sibling's fraction; the right
+ * child's tail is its node's.  A child with a tail below 2^-110 is rounded up
+ * by one unit instead: then it lies at most one unit above, which cannot
+ * carry into its digits because the tail is far below 1.  (Otherwise the
+ * tail is at least 2^-110 and the truncation error at most 2^-128 in the same
+ * units.)  One thread per split node, after the level's products. */
+struct TailFix {
+    uint64_t *l, *r;             /* the children's fractions (views) */
+    uint32_t pl, pr;
+    uint32_t parent, lslot, rslot;
+};
+
+__device__ __forceinline__ void add_ulp(uint64_t *x, uint32_t n)
+{
+    for (uint32_t k = 0; k < n; k++)
+        if (++x[k] != 0) break;
+}
+
+__global__ void __launch_bounds__(256) k_tail_fix(const TailFix *fx, size_t n, uint8_t *flag)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const TailFix f = fx[i];
+    const uint64_t r1 = f.r[f.pr - 1], r2 = f.pr >= 2 ? f.r[f.pr - 2] : 0;
+    const bool ltiny = r1 == 0 && r2 < ((uint64_t)1 << 18);
+    const bool rtiny = flag[f.parent] != 0;
+    flag[f.lslot] = ltiny;
+    flag[f.rslot] = rtiny;
+    if (ltiny) add_ulp(f.l, f.pl);
+    if (rtiny) add_ulp(f.r, f.pr);
+}
+
 /* ---- host side -------------------------------------------------------------- */
 
 namespace {
@@ -333,15 +370,306 @@ int newton_recip(Arena &ar, Scratch &sc, const uint64_t *Bn, size_t nB, size_t t
 }
 
 struct RNode {
-    const uint64_t *y;   /* fraction limbs (p of them, value y / 2^(64 p)) */
+    uint64_t *y;         /* fraction limbs (p of them, value y / 2^(64 p)) */
     size_t p;
     size_t m, pos;
     int depth;
+    uint32_t slot;       /* its tail flag (see k_tail_fix) */
 };
 
 size_t frac_limbs(size_t digits) { return (size_t)ceil((double)digits * 3.3219280948873623 / 64.0) + 2; }
 
 }  // namespace
+
+/* Digits 1..M of the fraction F / 2^(64 tgoal) (device, tgoal limbs),
+ * truncated, into d_digits[0, M) on the device: the fixed-point radix tree
+ * (a node with fraction y and m digits gives its left child y for the first
+ * k = 19 * 2^j digits and its right child frac(y * 10^k) for the rest; one
+ * batched multiply per depth), then 19-digit leaves in 128-bit arithmetic.
+ * Powers of 10 come from the arena; the tree's products from one of its own. */
+int fraction_digits(Arena &ar, uint64_t *F, size_t tgoal, size_t M, char *d_digits)
+{
+    int rc = NTT_OK;
+    Arena zr;   /* products of the conversion tree */
+    {
+        /* powers 10^(19 * 2^j) */
+        std::vector<uint64_t *> pw;
+        std::vector<size_t> pwn;
+        uint64_t *t1p = ar.take(2);
+        uint64_t ten19 = 10000000000000000000ull;
+        if (!t1p) rc = fail(NTT_ERR_NOMEM, "arena");
+        if (!rc) rc = copy_h2d(t1p, &ten19, 8);
+        pw.push_back(t1p);
+        pwn.push_back(1);
+        size_t leaves = (M + 18) / 19;
+        while (!rc && ((size_t)1 << pw.size()) < leaves) {
+            size_t n = pwn.back();
+            uint64_t *q = ar.take(2 * n);
+            if (!q) {
+                rc = fail(NTT_ERR_NOMEM, "arena");
+                break;
+            }
+            rc = ntt_sqr_device_async(q, pw.back(), n);
+            pw.push_back(q);
+            pwn.push_back(2 * n);
+        }
+        /* size the conversion tree's products, then plan it level by level */
+        if (!rc) {
+            size_t words = 0;
+            std::vector<std::pair<size_t, size_t>> lv{{tgoal, M}}, nx;   /* (p, m) */
+            while (!lv.empty()) {
+                nx.clear();
+                for (auto &pm : lv) {
+                    if (pm.second <= LEAF_DIGITS) continue;
+                    size_t nl = (pm.second + 18) / 19, K = 1;
+                    int j = 0;
+                    while (K * 2 < nl) K *= 2, j++;
+                    size_t k = 19 * K;
+                    words += ((pm.first + pwn[j]) + 1) & ~(size_t)1;
+                    nx.push_back({std::min(frac_limbs(k), pm.first), k});
+                    nx.push_back({std::min(frac_limbs(pm.second - k), pm.first), pm.second - k});
+                }
+                lv.swap(nx);
+            }
+            rc = zr.init(words + 64);
+        }
+        /* tail flags: the root's is 0 (beyond its M digits lies only the
+         * approximation error, which the caller's guard digits absorb) */
+        uint8_t *d_flag = nullptr;
+        const size_t max_nodes = 4 * ((M + LEAF_DIGITS - 1) / LEAF_DIGITS) + 8;
+        if (hipMalloc((void **)&d_flag, max_nodes) != hipSuccess) rc = fail(NTT_ERR_NOMEM, "tail flags");
+        const uint8_t zero = 0;
+        if (!rc) rc = copy_h2d(d_flag, &zero, 1);
+        uint32_t nslots = 1;
+        std::vector<std::vector<TailFix>> fixes;   /* host copies, kept until the final sync */
+        TailFix *d_fix = nullptr;                  /* every level's fixes, one after another */
+        size_t fix_used = 0;
+        const size_t fix_cap = max_nodes / 2 + 8;
+        if (!rc && hipMalloc((void **)&d_fix, fix_cap * sizeof(TailFix)) != hipSuccess)
+            rc = fail(NTT_ERR_NOMEM, "tail fixes");
+        std::vector<RNode> level{{F, tgoal, M, 0, 0, 0}};
+        std::vector<DigitLeaf> leafv;
+        while (!rc && !level.empty()) {
+            std::vector<RNode> next;
+            std::vector<ntt_mul_desc> descs;
+            fixes.emplace_back();
+            std::vector<TailFix> &fx = fixes.back();
+            for (const RNode &nd : level) {
+                if (nd.m <= LEAF_DIGITS) {
+                    if (nd.p > LEAF_LIMBS) {
+                        rc = fail(NTT_ERR_ARG, "leaf of %zu limbs (internal)", nd.p);
+                        break;
+                    }
+                    leafv.push_back({nd.y, nd.pos, (uint32_t)nd.m, (uint32_t)nd.p});
+                    continue;
+                }
+                size_t nl = (nd.m + 18) / 19, K = 1;
+                int j = 0;
+                while (K * 2 < nl) {
+                    K *= 2;
+                    j++;
+                }
+                size_t k = 19 * K;
+                size_t pl = std::min(frac_limbs(k), nd.p), pr = std::min(frac_limbs(nd.m - k), nd.p);
+                uint64_t *Z = zr.take(nd.p + pwn[j]);
+                if (!Z || nslots + 2 > max_nodes) {
+                    rc = fail(NTT_ERR_NOMEM, "arena");
+                    break;
+                }
+                const uint32_t ls = nslots++, rs = nslots++;
+                next.push_back({nd.y + (nd.p - pl), pl, k, nd.pos, nd.depth + 1, ls});
+                ntt_mul_desc dd{nd.y, pw[j], Z, nullptr, (uint32_t)nd.p, (uint32_t)pwn[j], 0,
+                                (uint32_t)(nd.p + pwn[j])};
+                descs.push_back(dd);
+                next.push_back({Z + (nd.p - pr), pr, nd.m - k, nd.pos + k, nd.depth + 1, rs});
+                fx.push_back({nd.y + (nd.p - pl), Z + (nd.p - pr), (uint32_t)pl, (uint32_t)pr, nd.slot, ls, rs});
+            }
+            if (!rc && !descs.empty()) rc = ntt_mul_batch_device_async(descs.data(), descs.size());
+            if (!rc && !fx.empty()) {
+                if (fix_used + fx.size() > fix_cap) rc = fail(NTT_ERR_NOMEM, "tail fixes (internal)");
+                TailFix *dst = d_fix + fix_used;
+                if (!rc && hipMemcpyAsync(dst, fx.data(), fx.size() * sizeof(TailFix), hipMemcpyHostToDevice,
+                                          g.stream) != hipSuccess)
+                    rc = fail(NTT_ERR_HIP, "tail fix upload");
+                if (!rc) {
+                    hipLaunchKernelGGL(k_tail_fix, dim3((unsigned)((fx.size() + 255) / 256)), dim3(256), 0, g.stream,
+                                       dst, fx.size(), d_flag);
+                    if (hipGetLastError() != hipSuccess) rc = fail(NTT_ERR_HIP, "k_tail_fix");
+                }
+                fix_used += fx.size();
+            }
+            level.swap(next);
+        }
+        DigitLeaf *d_leaves = nullptr;
+        if (!rc && hipMalloc((void **)&d_leaves, leafv.size() * sizeof(DigitLeaf)) != hipSuccess)
+            rc = fail(NTT_ERR_NOMEM, "leaves");
+        if (!rc && hipMemcpyAsync(d_leaves, leafv.data(), leafv.size() * sizeof(DigitLeaf), hipMemcpyHostToDevice,
+                                  g.stream) != hipSuccess)
+            rc = fail(NTT_ERR_HIP, "leaf upload");
+        if (!rc) {
+            hipLaunchKernelGGL(k_leaf_digits, dim3((unsigned)((leafv.size() + 255) / 256)), dim3(256), 0, g.stream,
+                               d_leaves, leafv.size(), d_digits);
+            if (hipGetLastError() != hipSuccess) rc = fail(NTT_ERR_HIP, "leaf kernel");
+        }
+        if (!rc) rc = sync_check();
+        if (rc) (void)hipStreamSynchronize(g.stream);
+        if (d_leaves) (void)hipFree(d_leaves);
+        if (d_fix) (void)hipFree(d_fix);
+        if (d_flag) (void)hipFree(d_flag);
+    }
+    zr.release();
+    return rc;
+}
+
+/* ---- integer to decimal on the device --------------------------------------
+ * The digits of N are the first D digits of the fraction f = (N + 1/2) / 10^D
+ * (D = 19 K >= digits of N), produced by the same fixed-point radix tree.
+ * The half unit makes the digits after position D read 5000... exactly, so
+ * the approximation error (far below 10^-(D+20)) shows up only as 5000.. or
+ * 4999.. in the guard digits and can never reach digit D; anything else in
+ * the guard digits, or a mismatch of the last 19 digits against N mod 10^19
+ * (computed on the host), is reported as an integrity error. */
+
+/* N mod 10^19 from the host limbs, top down. */
+static uint64_t mod_ten19(const uint64_t *h, size_t n)
+{
+    const uint64_t T = 10000000000000000000ull;
+    unsigned __int128 r = 0;
+    for (size_t i = n; i-- > 0;) r = ((r << 64) | h[i]) % T;
+    return (uint64_t)r;
+}
+
+__global__ void k_or_limb0(uint64_t *x, uint64_t v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) x[0] |= v;
+}
+
+extern "C" int ntt_to_decimal(const uint64_t *h, size_t n, char *out, size_t cap, size_t *len)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    if ((n && !h) || !out) return fail(NTT_ERR_ARG, "null buffer");
+    while (n && !h[n - 1]) n--;
+    if (n == 0) {
+        if (cap < 2) return fail(NTT_ERR_ARG, "out_cap too small");
+        out[0] = '0';
+        out[1] = 0;
+        if (len) *len = 1;
+        return NTT_OK;
+    }
+    const size_t bits = 64 * (n - 1) + 64 - __builtin_clzll(h[n - 1]);
+    const size_t dmax = (size_t)((double)bits * 0.30103) + 1;      /* >= digits of N (0.30103 > log10 2) */
+    if (cap < dmax + 1) return fail(NTT_ERR_ARG, "out_cap %zu below the %zu-digit bound + 1", cap, dmax);
+    const size_t K = (dmax + 18) / 19, D = 19 * K, G = 20, M = D + G;
+    const size_t tgoal = frac_limbs(M);
+    Arena ar;
+    Scratch sc;
+    char *d_digits = nullptr;
+    int rc = ar.init(40 * (n + tgoal) + 4096);
+    if (!rc) rc = sc.reserve(2 * (n + tgoal) + 64);
+    uint64_t *dN = rc ? nullptr : ar.take(n);
+    if (!rc && !dN) rc = fail(NTT_ERR_NOMEM, "arena");
+    if (!rc) rc = copy_h2d(dN, h, n * 8);
+    /* T = 10^D = prod of 10^(19 * 2^j) over the set bits j of K */
+    uint64_t *T = nullptr;
+    size_t nT = 0;
+    if (!rc) {
+        const uint64_t ten19 = 10000000000000000000ull;
+        uint64_t *pw = ar.take(2);
+        size_t npw = 1;
+        if (!pw) rc = fail(NTT_ERR_NOMEM, "arena");
+        if (!rc) rc = copy_h2d(pw, &ten19, 8);
+        for (size_t k = K; !rc && k; k >>= 1) {
+            if (k & 1) {
+                if (!T) {
+                    T = pw;
+                    nT = npw;
+                } else {
+                    uint64_t *q = ar.take(nT + npw);
+                    if (!q) rc = fail(NTT_ERR_NOMEM, "arena");
+                    if (!rc) rc = mul_dev(q, T, nT, pw, npw);
+                    T = q;
+                    nT += npw;
+                }
+            }
+            if (!rc && k > 1) {
+                uint64_t *q = ar.take(2 * npw);
+                if (!q) rc = fail(NTT_ERR_NOMEM, "arena");
+                if (!rc) rc = ntt_sqr_device_async(q, pw, npw);
+                pw = q;
+                npw *= 2;
+            }
+        }
+        /* the powers carry up to ~1.5 % leading zero limbs */
+        if (!rc) rc = dev_trim_len(T, nT, nT / 48 + 8, &nT);
+    }
+    /* normalized divisor 2 * 10^D and its reciprocal */
+    uint64_t *Bn = nullptr, *X = nullptr;
+    size_t nB = 0;
+    int sh = 0;
+    if (!rc) {
+        uint64_t *M2 = ar.take(nT + 1);
+        Bn = ar.take(nT + 1);
+        if (!M2 || !Bn) rc = fail(NTT_ERR_NOMEM, "arena");
+        if (!rc) rc = dev_shl(M2, nT + 1, T, nT, 0, 1);
+        uint64_t top = 0;
+        if (!rc) rc = copy_d2h(&top, M2 + nT, 8);
+        nB = top ? nT + 1 : nT;
+        if (!rc && !top) rc = copy_d2h(&top, M2 + nT - 1, 8);
+        if (!rc) {
+            sh = __builtin_clzll(top);
+            rc = dev_shl(Bn, nB, M2, nB, 0, sh);
+        }
+    }
+    if (!rc) rc = newton_recip(ar, sc, Bn, nB, tgoal, &X);
+    /* F = ((2N + 1) << sh) * X >> 64 nB: f to tgoal limbs */
+    uint64_t *F = nullptr;
+    if (!rc) {
+        uint64_t *Y = ar.take(n + 2), *Fz = ar.take(n + 2 + tgoal + 1);
+        if (!Y || !Fz) rc = fail(NTT_ERR_NOMEM, "arena");
+        if (!rc) rc = dev_shl(Y, n + 2, dN, n, (size_t)(sh + 1) >> 6, (sh + 1) & 63);   /* 2N << sh */
+        if (!rc) {
+            hipLaunchKernelGGL(k_or_limb0, dim3(1), dim3(64), 0, g.stream, Y, (uint64_t)1 << sh);
+            if (hipGetLastError() != hipSuccess) rc = fail(NTT_ERR_HIP, "k_or_limb0");
+        }
+        if (!rc) rc = mul_dev(Fz, Y, n + 2, X, tgoal + 1);
+        F = Fz ? Fz + nB : nullptr;
+    }
+    if (!rc && hipMalloc((void **)&d_digits, M + 64) != hipSuccess) rc = fail(NTT_ERR_NOMEM, "digit buffer");
+    if (!rc) rc = fraction_digits(ar, F, tgoal, M, d_digits);
+    std::vector<char> dig;
+    if (!rc) {
+        dig.resize(M);
+        rc = copy_d2h(dig.data(), d_digits, M);
+    }
+    if (!rc) {
+        /* guard digits: 5 then 0s, or 4 then 9s */
+        const char g0 = dig[D], fill = g0 == '5' ? '0' : '9';
+        bool ok = g0 == '5' || g0 == '4';
+        for (size_t k = D + 1; ok && k < D + 12; k++) ok = dig[k] == fill;
+        if (!ok) rc = fail(NTT_ERR_INTEGRITY, "decimal conversion: guard digits %.12s", dig.data() + D);
+    }
+    if (!rc) {
+        size_t z = 0;
+        while (z + 1 < D && dig[z] == '0') z++;
+        const size_t nd = D - z;
+        /* the last 19 digits against N mod 10^19 */
+        uint64_t r = 0;
+        for (size_t k = D >= 19 ? D - 19 : 0; k < D; k++) r = r * 10 + (uint64_t)(dig[k] - '0');
+        if (r != mod_ten19(h, n) || nd > dmax || dig[z] == '0')
+            rc = fail(NTT_ERR_INTEGRITY, "decimal conversion: digits disagree with N mod 10^19");
+        if (!rc) {
+            memcpy(out, dig.data() + z, nd);
+            out[nd] = 0;
+            if (len) *len = nd;
+        }
+    }
+    if (d_digits) (void)hipFree(d_digits);
+    if (rc) (void)hipStreamSynchronize(g.stream);
+    sc.release();
+    ar.release();
+    return rc;
+}
 
 extern "C" int ntt_compute_e(long n_digits, int batched_tree, char *out, size_t out_cap, ntt_e_stats *st)
 {
@@ -423,105 +751,13 @@ extern "C" int ntt_compute_e(long n_digits, int batched_tree, char *out, size_t 
         if (!rc) rc = mul_dev(Fz, M, nP + 1, X, tgoal + 1);
         if (!rc) rc = sync_check();
     }
-    const uint64_t *F = Fz ? Fz + nB : nullptr;   /* tgoal limbs of fraction */
+    uint64_t *F = Fz ? Fz + nB : nullptr;   /* tgoal limbs of fraction */
     double t3 = now_s();
     stats.t_div = t3 - t2;
     /* 4. radix conversion of the n_total - 1 fractional digits */
     char *d_digits = nullptr;
-    Arena zr;   /* products of the conversion tree */
     if (!rc && hipMalloc((void **)&d_digits, M + 64) != hipSuccess) rc = fail(NTT_ERR_NOMEM, "digit buffer");
-    if (!rc) {
-        /* powers 10^(19 * 2^j) */
-        std::vector<uint64_t *> pw;
-        std::vector<size_t> pwn;
-        uint64_t *t1p = ar.take(2);
-        uint64_t ten19 = 10000000000000000000ull;
-        if (!t1p) rc = fail(NTT_ERR_NOMEM, "arena");
-        if (!rc) rc = copy_h2d(t1p, &ten19, 8);
-        pw.push_back(t1p);
-        pwn.push_back(1);
-        size_t leaves = (M + 18) / 19;
-        while (!rc && ((size_t)1 << pw.size()) < leaves) {
-            size_t n = pwn.back();
-            uint64_t *q = ar.take(2 * n);
-            if (!q) {
-                rc = fail(NTT_ERR_NOMEM, "arena");
-                break;
-            }
-            rc = ntt_sqr_device_async(q, pw.back(), n);
-            pw.push_back(q);
-            pwn.push_back(2 * n);
-        }
-        /* size the conversion tree's products, then plan it level by level */
-        if (!rc) {
-            size_t words = 0;
-            std::vector<std::pair<size_t, size_t>> lv{{tgoal, M}}, nx;   /* (p, m) */
-            while (!lv.empty()) {
-                nx.clear();
-                for (auto &pm : lv) {
-                    if (pm.second <= LEAF_DIGITS) continue;
-                    size_t nl = (pm.second + 18) / 19, K = 1;
-                    int j = 0;
-                    while (K * 2 < nl) K *= 2, j++;
-                    size_t k = 19 * K;
-                    words += ((pm.first + pwn[j]) + 1) & ~(size_t)1;
-                    nx.push_back({std::min(frac_limbs(k), pm.first), k});
-                    nx.push_back({std::min(frac_limbs(pm.second - k), pm.first), pm.second - k});
-                }
-                lv.swap(nx);
-            }
-            rc = zr.init(words + 64);
-        }
-        std::vector<RNode> level{{F, tgoal, M, 0, 0}};
-        std::vector<DigitLeaf> leafv;
-        while (!rc && !level.empty()) {
-            std::vector<RNode> next;
-            std::vector<ntt_mul_desc> descs;
-            for (const RNode &nd : level) {
-                if (nd.m <= LEAF_DIGITS) {
-                    if (nd.p > LEAF_LIMBS) {
-                        rc = fail(NTT_ERR_ARG, "leaf of %zu limbs (internal)", nd.p);
-                        break;
-                    }
-                    leafv.push_back({nd.y, nd.pos, (uint32_t)nd.m, (uint32_t)nd.p});
-                    continue;
-                }
-                size_t nl = (nd.m + 18) / 19, K = 1;
-                int j = 0;
-                while (K * 2 < nl) {
-                    K *= 2;
-                    j++;
-                }
-                size_t k = 19 * K;
-                size_t pl = std::min(frac_limbs(k), nd.p), pr = std::min(frac_limbs(nd.m - k), nd.p);
-                next.push_back({nd.y + (nd.p - pl), pl, k, nd.pos, nd.depth + 1});
-                uint64_t *Z = zr.take(nd.p + pwn[j]);
-                if (!Z) {
-                    rc = fail(NTT_ERR_NOMEM, "arena");
-                    break;
-                }
-                ntt_mul_desc dd{nd.y, pw[j], Z, nullptr, (uint32_t)nd.p, (uint32_t)pwn[j], 0,
-                                (uint32_t)(nd.p + pwn[j])};
-                descs.push_back(dd);
-                next.push_back({Z + (nd.p - pr), pr, nd.m - k, nd.pos + k, nd.depth + 1});
-            }
-            if (!rc && !descs.empty()) rc = ntt_mul_batch_device_async(descs.data(), descs.size());
-            level.swap(next);
-        }
-        DigitLeaf *d_leaves = nullptr;
-        if (!rc && hipMalloc((void **)&d_leaves, leafv.size() * sizeof(DigitLeaf)) != hipSuccess)
-            rc = fail(NTT_ERR_NOMEM, "leaves");
-        if (!rc && hipMemcpyAsync(d_leaves, leafv.data(), leafv.size() * sizeof(DigitLeaf), hipMemcpyHostToDevice,
-                                  g.stream) != hipSuccess)
-            rc = fail(NTT_ERR_HIP, "leaf upload");
-        if (!rc) {
-            hipLaunchKernelGGL(k_leaf_digits, dim3((unsigned)((leafv.size() + 255) / 256)), dim3(256), 0, g.stream,
-                               d_leaves, leafv.size(), d_digits);
-            if (hipGetLastError() != hipSuccess) rc = fail(NTT_ERR_HIP, "leaf kernel");
-        }
-        if (!rc) rc = sync_check();
-        if (d_leaves) (void)hipFree(d_leaves);
-    }
+    if (!rc) rc = fraction_digits(ar, F, tgoal, M, d_digits);
     std::vector<char> digits;
     if (!rc) {
         digits.resize(M);
@@ -549,7 +785,6 @@ extern "C" int ntt_compute_e(long n_digits, int batched_tree, char *out, size_t 
     if (tree) ntt_split_tree_destroy(tree);
     sc.release();
     ar.release();
-    zr.release();
     stats.t_total = now_s() - t0;
     stats.products = g.dispatches - disp0;
     if (st) *st = stats;

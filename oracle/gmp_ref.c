@@ -106,3 +106,24 @@ int gmpref_e_digits(long n_digits, char *out)
     mpz_clears(P, B, num, t, NULL);
     return rc;
 }
+
+/* Decimal digits of the n-limb integer d (mpz_get_str), NUL-terminated, into
+ * out (cap bytes); returns the digit count, or -1 when cap is too small. */
+long gmpref_to_decimal(const uint64_t *d, size_t n, char *out, size_t cap)
+{
+    mpz_t z;
+    mpz_init(z);
+    imp(z, d, n);
+    char *s = mpz_get_str(NULL, 10, z);
+    size_t len = strlen(s);
+    long rc = -1;
+    if (len + 1 <= cap) {
+        memcpy(out, s, len + 1);
+        rc = (long)len;
+    }
+    void (*freefunc)(void *, size_t);
+    mp_get_memory_functions(NULL, NULL, &freefunc);
+    freefunc(s, len + 1);
+    mpz_clear(z);
+    return rc;
+}

@@ -87,6 +87,8 @@ def gmp():
                                           ctypes.POINTER(_u64p), ctypes.POINTER(ctypes.c_size_t)]
         L.gmpref_e_digits.restype = ctypes.c_int
         L.gmpref_e_digits.argtypes = [ctypes.c_long, ctypes.c_char_p]
+        L.gmpref_to_decimal.restype = ctypes.c_long
+        L.gmpref_to_decimal.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
         _gmp = L
     return _gmp
 
@@ -186,3 +188,14 @@ def gmp_e_digits(n_digits):
     if gmp().gmpref_e_digits(n_digits, buf) != 0:
         raise RuntimeError("gmpref_e_digits failed")
     return buf.value.decode()
+
+
+def gmp_to_decimal(limbs):
+    """Decimal string of the integer in the u64 limb array (GMP mpz_get_str)."""
+    a = np.ascontiguousarray(limbs, dtype=np.uint64)
+    cap = len(a) * 20 + 2
+    buf = ctypes.create_string_buffer(cap)
+    n = gmp().gmpref_to_decimal(_ptr(a), len(a), buf, cap)
+    if n < 0:
+        raise RuntimeError("gmpref_to_decimal: buffer too small")
+    return buf.raw[:n].decode()

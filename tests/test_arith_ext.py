@@ -216,6 +216,43 @@ def test_gpu_division_exact(gpu_lib):
 
 
 @pytest.mark.gpu
+def test_gpu_decimal_device_path(gpu_lib):
+    """Above 256 limbs bigint_to_decimal runs ntt_to_decimal (the fraction
+    (N + 1/2) / 10^D through the device radix tree): powers of ten and of two
+    and their neighbours (all-9 and all-0 digit runs), the threshold, and
+    random values, against str(); then a 2*10^6-digit value against GMP."""
+    gpu_lib.base_convert_init(20)
+    try:
+        _device_decimal_cases(gpu_lib)
+    finally:
+        gpu_lib.base_convert_teardown()      # later tests start from an empty cache
+
+
+def _device_decimal_cases(gpu_lib):
+    rng = random.Random(24)
+    cases = []
+    for d in (4932, 4933, 4940, 5000, 19 * 300, 19 * 300 + 1, 12345, 77777):
+        cases += [10 ** d - 1, 10 ** d, 10 ** d + 1, 5 * 10 ** (d - 1), 10 ** d // 2, rng.randrange(10 ** (d - 1), 10 ** d)]
+    for bits in (16384, 16385, 65536, 65537, 200003):
+        cases += [(1 << bits) - 1, 1 << bits, (1 << bits) + 1, rng.getrandbits(bits) | (1 << (bits - 1))]
+    for v in cases:
+        assert to_dec(gpu_lib, v) == str(v), (v.bit_length(), len(str(v)))
+    L = gpu_lib
+    L.ntt_to_decimal.argtypes = [_u64p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t,
+                                 ctypes.POINTER(ctypes.c_size_t)]
+    import _oracle as O
+    limbs = O.splitmix(77, 104000)                     # about 2.0e6 digits
+    cap = int(64 * len(limbs) * 0.30103) + 2
+    buf = ctypes.create_string_buffer(cap)
+    n = ctypes.c_size_t()
+    assert L.ntt_to_decimal(limbs.ctypes.data_as(_u64p), len(limbs), buf, cap, ctypes.byref(n)) == 0
+    assert buf.raw[:n.value].decode() == O.gmp_to_decimal(limbs)
+    zero = np.zeros(3, np.uint64)
+    assert L.ntt_to_decimal(zero.ctypes.data_as(_u64p), 3, buf, cap, ctypes.byref(n)) == 0
+    assert buf.raw[:n.value] == b"0"
+
+
+@pytest.mark.gpu
 def test_gpu_decimal_conversion(gpu_lib):
     rng = random.Random(23)
     check_decimal(gpu_lib, rng, 20000, 15)
