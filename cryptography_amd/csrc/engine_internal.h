@@ -34,6 +34,11 @@ struct Engine {
     /* host staging for the host-buffer API */
     uint64_t *d_a = nullptr, *d_b = nullptr, *d_c = nullptr;
     size_t d_a_words = 0, d_b_words = 0, d_c_words = 0;
+    /* two pinned host buffers for chunked transfers of large operands
+     * (allocated on first use), each with the event of its last copy */
+    void *pin[2] = {nullptr, nullptr};
+    size_t pin_bytes = 0;
+    hipEvent_t pin_ev[2] = {};
     int threshold = 64;
     /* largest sub-transform (log2) per pass, per transform length 2^l;
      * 0 = default (FT_MAX_LOGS).  Set by the autotuner. */

@@ -156,3 +156,21 @@ def test_tile_batch_parity(tiles):
         assert np.array_equal(E.sqr(b), O.gmp_mul(b, b))
     finally:
         E.engine.set_tile_batch(old)
+
+
+def test_host_api_staged_transfers():
+    """Host-buffer products above the staging threshold go through pinned
+    64 MiB pieces (several per operand here): the result equals the
+    device-resident product bit for bit, and the product modulo a prime."""
+    n = 9 << 20                                      # 72 MiB per operand, 144 MiB product
+    a = O.splitmix(91, n)
+    b = O.splitmix(92, n)
+    got = E.mul(a, b)
+    da, db, dc = E.DeviceBuffer.from_array(a), E.DeviceBuffer.from_array(b), E.DeviceBuffer(2 * n)
+    E.mul_device(dc, da, n, db, n)
+    assert np.array_equal(got, dc.download())
+    q = (1 << 61) - 1
+    A, B, C = (int.from_bytes(x.tobytes(), "little") for x in (a, b, got))
+    assert C % q == (A % q) * (B % q) % q
+    sq = E.sqr(a)
+    assert int.from_bytes(sq.tobytes(), "little") % q == (A % q) * (A % q) % q
