@@ -250,7 +250,7 @@ void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t
 /* Enqueue one size class: nprod products, transform length 2^logL, all on
  * `st`.  d_desc == nullptr means the single product `one`. */
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
-                      uint32_t max_nout, int square, hipStream_t st, int residues_in)
+                      uint32_t max_nout, int square, hipStream_t st, int residues_in, int shared_b)
 {
     const size_t L = (size_t)1 << logL;
     const size_t len = row_len(logL, nprod);
@@ -294,17 +294,33 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     };
     /* residues_in: the rows already hold the residues (polynomial products),
      * so no pass reads the operands' limbs */
+    /* a shared b (every product multiplies by one value) is transformed once,
+     * as product 0's b rows, which the fused pass then reads for all; that
+     * needs whole tiles per product (L >= 4096) and separate passes */
+    const bool share = shared_b && !square && !residues_in && pp.npass >= 2 && logL >= 12 && nprod > 1;
+    const int fwd_ops = share ? 1 : nops;
     uint32_t done = 0;
     for (int i = 0; i < pp.npass - 1; i++) {
         geom(i, done);
         pa.first = i == 0 && !residues_in;
-        HIP_TRY(launch_fast_pass(pp.logS[i], pa.first ? 3 : 0, pa, ntiles, nops, st));
+        HIP_TRY(launch_fast_pass(pp.logS[i], pa.first ? 3 : 0, pa, ntiles, fwd_ops, st));
         HIP_TRY(mark("k_fast_pass<fwd>"));
+        if (share) {   /* b of product 0 only */
+            PassArgs pb = pa;
+            pb.op0 = 1;
+            pb.nprod = 1;
+            pb.ntiles = (uint32_t)(L / FT_TILE);
+            pb.tpb = 1;
+            HIP_TRY(launch_fast_pass(pp.logS[i], pb.first ? 3 : 0, pb, pb.ntiles, 1, st));
+            HIP_TRY(mark("k_fast_pass<fwd b>"));
+        }
         done += pp.logS[i];
     }
     geom(pp.npass - 1, done);
     pa.first = pp.npass == 1 && !residues_in;
+    pa.b_tiles = share ? (uint32_t)(L / FT_TILE) : 0;
     HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], pa.first ? 4 : 2, pa, ntiles, nops, st));
+    pa.b_tiles = 0;
     HIP_TRY(mark("k_fast_pass<mid>"));
     for (int i = pp.npass - 2; i >= 0; i--) {
         done -= pp.logS[i];
@@ -489,6 +505,13 @@ int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
     bp.res_words = bp.scratch = 0;
     for (auto &c : bp.classes) {
         if (c.logL <= 0) continue;
+        const MulDesc &d0 = bp.sorted[c.first];
+        bool same = c.count > 1;
+        for (uint32_t i = 1; same && i < c.count; i++) {
+            const MulDesc &d = bp.sorted[c.first + i];
+            same = d.b == d0.b && d.nb == d0.nb;
+        }
+        c.shared_b = same;
         size_t rw, sb;
         class_needs(c.logL, c.count, c.max_nout, 0, rw, sb);
         bp.res_words = std::max(bp.res_words, rw);
@@ -519,7 +542,7 @@ int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hip
         } else if (c.logL == 0) {
             rc = enqueue_schoolbook(dd, MulDesc{}, c.count, st, ws.flag);
         } else {
-            rc = enqueue_ntt_class(ws, dd, MulDesc{}, c.count, c.logL, c.max_nout, 0, st);
+            rc = enqueue_ntt_class(ws, dd, MulDesc{}, c.count, c.logL, c.max_nout, 0, st, 0, c.shared_b);
         }
         if (rc) return rc;
     }

@@ -686,7 +686,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
                 lds[LP(q)] = first_in(pa, 1, ft_addr(LOGS, g, t, q, m, r));
             }
         } else {
-            tile_dma<LOGS>(pa, xb, g, t, lds);
+            tile_dma<LOGS>(pa, xb, g, pa.b_tiles ? t % pa.b_tiles : t, lds);
             WAIT_VMCNT(0);
         }
         lds_sync();
@@ -748,7 +748,7 @@ __global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
 
     static_assert((1 << LOGS) <= TW_LDS_MAX, "stage twiddles exceed LDS table");
     TwPair *twf = tws, *twi = tws + (NTW - 1) * (1 << LOGS);
-    const uint32_t pi = blockIdx.y, op = blockIdx.z;
+    const uint32_t pi = blockIdx.y, op = blockIdx.z + pa.op0;
     uint64_t *xa = pa.res + ((size_t)op * NTT_NPRIMES + pi) * pa.len;
     const uint64_t *xb = pa.res + ((size_t)NTT_NPRIMES + pi) * pa.len;
     if (pi == 0) pass_body<0, LOGS, KIND>(pa, xa, xb, op, lds, twf, twi);

@@ -54,6 +54,9 @@ struct PassArgs {
     uint32_t tpb;               /* forward / inverse passes: tiles per block */
     int first;                  /* read the operands' 16-bit coefficients */
     int square;                 /* b == a: one forward transform */
+    uint32_t op0;               /* forward passes: operand of grid z = 0 (1: b only) */
+    uint32_t b_tiles;           /* fused pass: every product shares product 0's b
+                                   spectrum, whose tiles are t mod b_tiles (0: own b) */
     uint64_t scale[3], scale_p[3];   /* 2^64 / L mod p and Shoup partners (middle pass) */
 };
 

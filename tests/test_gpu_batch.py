@@ -114,3 +114,44 @@ def test_batch_overflow_is_reported():
         assert np.array_equal(E.mul(x, x), O.gmp_mul(x, x))
     finally:
         pool.close()
+
+
+@pytest.mark.parametrize("nb,na_list", [
+    (1500, [1400, 1500, 1600, 1550, 1450]),          # L = 2^14: b transformed once, shared
+    (40000, [30000, 35000, 39000]),                  # L = 2^19
+    (200, [150, 180, 210, 250]),                     # L = 2^11: single tiles hold several products, no sharing
+])
+def test_batch_shared_operand(nb, na_list):
+    """Products of one size class that all multiply by the same b transform b
+    once (the fused pass reads product 0's spectrum for all of them); each
+    result equals GMP, also with a trailing product of another b that keeps
+    its class unshared."""
+    pool = EN.DevicePool(8 * (nb + max(na_list)) * (len(na_list) + 2) + 4096)
+    try:
+        b = O.splitmix(5150, nb)
+        ob = pool.take(nb)
+        pool.upload(ob, b)
+        items, host = [], []
+        for i, na in enumerate(na_list):
+            a = O.splitmix(5151 + i, na)
+            oa, oc = pool.take(na), pool.take(na + nb)
+            pool.upload(oa, a)
+            items.append((pool.ptr(oa), na, pool.ptr(ob), nb, pool.ptr(oc), na + nb, None, 0))
+            host.append((a, b, oc))
+        EN.mul_batch_device(EN.make_descs(items))
+        for a, bb, oc in host:
+            assert np.array_equal(pool.download(oc, len(a) + len(bb)), O.gmp_mul(a, bb))
+        # the same class with one product of a different b: no sharing
+        b2 = O.splitmix(5160, nb)
+        ob2 = pool.take(nb)
+        pool.upload(ob2, b2)
+        a = host[0][0]
+        oa, oc = pool.take(len(a)), pool.take(len(a) + nb)
+        pool.upload(oa, a)
+        items.append((pool.ptr(oa), len(a), pool.ptr(ob2), nb, pool.ptr(oc), len(a) + nb, None, 0))
+        EN.mul_batch_device(EN.make_descs(items))
+        assert np.array_equal(pool.download(oc, len(a) + nb), O.gmp_mul(a, b2))
+        for a_, bb, oc_ in host:
+            assert np.array_equal(pool.download(oc_, len(a_) + len(bb)), O.gmp_mul(a_, bb))
+    finally:
+        pool.close()
