@@ -114,7 +114,9 @@ struct BatchPlan {
     struct Class {
         int logL;            /* 0: schoolbook */
         uint32_t first, count, max_nout;
-        int shared_b = 0;    /* every product has the same b (pointer and length) */
+        /* products i and i + b_period have the same b (pointer and length):
+         * 1 when all share one, count when none repeats */
+        uint32_t b_period = 0;
     };
     std::vector<MulDesc, uninit_alloc<MulDesc>> sorted;   /* descriptors grouped by class */
     std::vector<Class> classes;
@@ -143,7 +145,7 @@ uint32_t pad_log_for(int logL);
  * residues_in: then the rows hold residues on entry and on exit). */
 void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t &res_words, size_t &scratch);
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
-                      uint32_t max_nout, int square, hipStream_t st, int residues_in = 0, int shared_b = 0);
+                      uint32_t max_nout, int square, hipStream_t st, int residues_in = 0, uint32_t b_period = 0);
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);
 
 /* Host threads for planning work (descriptor sorting, tree construction):

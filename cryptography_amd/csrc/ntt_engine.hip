@@ -250,7 +250,7 @@ void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t
 /* Enqueue one size class: nprod products, transform length 2^logL, all on
  * `st`.  d_desc == nullptr means the single product `one`. */
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
-                      uint32_t max_nout, int square, hipStream_t st, int residues_in, int shared_b)
+                      uint32_t max_nout, int square, hipStream_t st, int residues_in, uint32_t b_period)
 {
     const size_t L = (size_t)1 << logL;
     const size_t len = row_len(logL, nprod);
@@ -294,10 +294,12 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     };
     /* residues_in: the rows already hold the residues (polynomial products),
      * so no pass reads the operands' limbs */
-    /* a shared b (every product multiplies by one value) is transformed once,
-     * as product 0's b rows, which the fused pass then reads for all; that
-     * needs whole tiles per product (L >= 4096) and separate passes */
-    const bool share = shared_b && !square && !residues_in && pp.npass >= 2 && logL >= 12 && nprod > 1;
+    /* b repeating with period P < nprod (products i and i + P share b) is
+     * transformed for the first P products only; the fused pass reads tile
+     * t mod (P L / 4096) of those rows.  Needs whole tiles per product
+     * (L >= 4096) and separate passes. */
+    const bool share = b_period && b_period < nprod && nprod % b_period == 0 && !square && !residues_in &&
+                       pp.npass >= 2 && logL >= 12;
     const int fwd_ops = share ? 1 : nops;
     uint32_t done = 0;
     for (int i = 0; i < pp.npass - 1; i++) {
@@ -305,12 +307,13 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         pa.first = i == 0 && !residues_in;
         HIP_TRY(launch_fast_pass(pp.logS[i], pa.first ? 3 : 0, pa, ntiles, fwd_ops, st));
         HIP_TRY(mark("k_fast_pass<fwd>"));
-        if (share) {   /* b of product 0 only */
+        if (share) {   /* b of the first b_period products */
             PassArgs pb = pa;
             pb.op0 = 1;
-            pb.nprod = 1;
-            pb.ntiles = (uint32_t)(L / FT_TILE);
-            pb.tpb = 1;
+            pb.nprod = b_period;
+            pb.ntiles = (uint32_t)(b_period * (L / FT_TILE));
+            pb.tpb = (uint32_t)std::min<size_t>(std::max<size_t>((size_t)pb.ntiles * NTT_NPRIMES / 4096, 1),
+                                                (size_t)g.tile_batch);
             HIP_TRY(launch_fast_pass(pp.logS[i], pb.first ? 3 : 0, pb, pb.ntiles, 1, st));
             HIP_TRY(mark("k_fast_pass<fwd b>"));
         }
@@ -318,7 +321,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     }
     geom(pp.npass - 1, done);
     pa.first = pp.npass == 1 && !residues_in;
-    pa.b_tiles = share ? (uint32_t)(L / FT_TILE) : 0;
+    pa.b_tiles = share ? (uint32_t)(b_period * (L / FT_TILE)) : 0;
     HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], pa.first ? 4 : 2, pa, ntiles, nops, st));
     pa.b_tiles = 0;
     HIP_TRY(mark("k_fast_pass<mid>"));
@@ -505,13 +508,18 @@ int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
     bp.res_words = bp.scratch = 0;
     for (auto &c : bp.classes) {
         if (c.logL <= 0) continue;
-        const MulDesc &d0 = bp.sorted[c.first];
-        bool same = c.count > 1;
-        for (uint32_t i = 1; same && i < c.count; i++) {
-            const MulDesc &d = bp.sorted[c.first + i];
-            same = d.b == d0.b && d.nb == d0.nb;
-        }
-        c.shared_b = same;
+        /* b repeating with period 1 (one b for all) or count/2 (pairs i and
+         * i + count/2, as the product tree lays out B = B_L B_R and
+         * P = P_L B_R + P_R) */
+        const MulDesc *cd = bp.sorted.data() + c.first;
+        auto period_ok = [&](uint32_t per) {
+            for (uint32_t i = per; i < c.count; i++)
+                if (cd[i].b != cd[i % per].b || cd[i].nb != cd[i % per].nb) return false;
+            return true;
+        };
+        c.b_period = c.count;
+        if (c.count > 1 && period_ok(1)) c.b_period = 1;
+        else if (c.count > 1 && c.count % 2 == 0 && period_ok(c.count / 2)) c.b_period = c.count / 2;
         size_t rw, sb;
         class_needs(c.logL, c.count, c.max_nout, 0, rw, sb);
         bp.res_words = std::max(bp.res_words, rw);
@@ -542,7 +550,7 @@ int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hip
         } else if (c.logL == 0) {
             rc = enqueue_schoolbook(dd, MulDesc{}, c.count, st, ws.flag);
         } else {
-            rc = enqueue_ntt_class(ws, dd, MulDesc{}, c.count, c.logL, c.max_nout, 0, st, 0, c.shared_b);
+            rc = enqueue_ntt_class(ws, dd, MulDesc{}, c.count, c.logL, c.max_nout, 0, st, 0, c.b_period);
         }
         if (rc) return rc;
     }

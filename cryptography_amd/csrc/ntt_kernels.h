@@ -55,8 +55,8 @@ struct PassArgs {
     int first;                  /* read the operands' 16-bit coefficients */
     int square;                 /* b == a: one forward transform */
     uint32_t op0;               /* forward passes: operand of grid z = 0 (1: b only) */
-    uint32_t b_tiles;           /* fused pass: every product shares product 0's b
-                                   spectrum, whose tiles are t mod b_tiles (0: own b) */
+    uint32_t b_tiles;           /* fused pass: b spectra repeat, tile t's is at
+                                   t mod b_tiles (0: every product has its own) */
     uint64_t scale[3], scale_p[3];   /* 2^64 / L mod p and Shoup partners (middle pass) */
 };
 

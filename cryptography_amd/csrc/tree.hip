@@ -223,7 +223,7 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
             } else {
                 capB[i] = cap_limbs_B(l, r);
                 capP[i] = capB[i] + 1;
-                np[h] += 2;
+                np[h] += 1;
             }
             words += capP[i] + capB[i];
         }
@@ -268,19 +268,21 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
     if (!rc && hipMalloc((void **)&t->d_leaves, leaves.size() * sizeof(LeafDesc)) != hipSuccess)
         rc = fail(NTT_ERR_NOMEM, "leaf descriptors");
     if (!rc) rc = copy_h2d(t->d_leaves, leaves.data(), leaves.size() * sizeof(LeafDesc));
-    /* one batch per height: B = B_L*B_R and P = P_L*B_R + P_R, in node order */
+    /* one batch per height: the B = B_L*B_R products of its nodes in node
+     * order, then their P = P_L*B_R + P_R products in the same order, so
+     * products i and i + n/2 share B_R (transformed once, see plan_batch) */
     std::vector<std::vector<ntt_mul_desc, uninit_alloc<ntt_mul_desc>>> per(H + 1);
-    for (int h = 1; h <= H; h++) per[h].resize(per_n[h]);
+    for (int h = 1; h <= H; h++) per[h].resize(2 * per_n[h]);
     uint64_t *A = t->arena;
     par_chunks(N, [&](size_t i0, size_t i1, unsigned w) {
         uint64_t *at = &c_prod[(size_t)w * (HMAX + 1)];
         for (size_t i = i0; i < i1; i++) {
             const int h = height[i];
             if (!h) continue;
-            const size_t l = kid[i], r = l + 1;
-            per[h][at[h]++] = ntt_mul_desc{A + offB[l], A + offB[r], A + offB[i], nullptr, capB[l], capB[r], 0, capB[i]};
-            per[h][at[h]++] = ntt_mul_desc{A + offP[l], A + offB[r], A + offP[i], A + offP[r], capP[l], capB[r], capP[r],
-                                           capP[i]};
+            const size_t l = kid[i], r = l + 1, k = at[h]++;
+            per[h][k] = ntt_mul_desc{A + offB[l], A + offB[r], A + offB[i], nullptr, capB[l], capB[r], 0, capB[i]};
+            per[h][per_n[h] + k] = ntt_mul_desc{A + offP[l], A + offB[r], A + offP[i], A + offP[r], capP[l], capB[r],
+                                                capP[r], capP[i]};
         }
     });
     t->levels.resize(H + 1);
