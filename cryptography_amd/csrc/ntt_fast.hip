@@ -17,10 +17,15 @@
  *   - the last forward pass of both operands, the pointwise product (with the
  *     1/L scale) and the first inverse pass are one kernel (kind 2): both
  *     spectra of a tile are formed, multiplied and transformed back without
- *     leaving the chip.
- * A tile is 4096 residues (32 KB of LDS) worked on by 512 threads; rows of a
- * size class are padded to a multiple of 4096 residues, and residue index a
- * lives at res_at(a, pad_log) (a pad after every first-pass row).
+ *     leaving the chip;
+ *   - tiles reach LDS by DMA (global_load_lds_dwordx4), no registers, with
+ *     the stage twiddles written and the cross-twiddle bases looked up while
+ *     the transfer is in flight; block barriers wait for LDS work only, so
+ *     stores stay in flight across them.
+ * A tile is 4096 residues (32 KB of LDS) worked on by 512 threads: with 8 KB
+ * of stage twiddles, three blocks share a CU (registers bound the fourth).
+ * Rows of a size class are padded to a multiple of 4096 residues, and residue
+ * index a lives at res_at(a, pad_log) (a pad after every first-pass row).
  *
  * Reference counterparts: ntt/lib/ntt_kernel.hip:625-675 (stockham_kernel),
  * :739-778 (stockham_kernel_had: Hadamard fused into the inverse's first pass),
