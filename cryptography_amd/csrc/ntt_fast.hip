@@ -18,10 +18,12 @@
  *     1/L scale) and the first inverse pass are one kernel (kind 2): both
  *     spectra of a tile are formed, multiplied and transformed back without
  *     leaving the chip;
- *   - tiles reach LDS by DMA (global_load_lds_dwordx4), no registers, with
- *     the stage twiddles written and the cross-twiddle bases looked up while
- *     the transfer is in flight; block barriers wait for LDS work only, so
- *     stores stay in flight across them.
+ *   - the forward column pass, the in-round inverse and the fused pass take
+ *     their tiles into LDS by DMA (global_load_lds_dwordx4, no registers),
+ *     writing the stage twiddles and looking up the cross-twiddle bases while
+ *     the transfer is in flight (the first pass expands operand limbs, the
+ *     load-chain inverse twiddles in registers: both load through VGPRs);
+ *     block barriers wait for LDS work only, so stores stay in flight.
  * A tile is 4096 residues (32 KB of LDS) worked on by 512 threads: with 8 KB
  * of stage twiddles, three blocks share a CU (registers bound the fourth).
  * Rows of a size class are padded to a multiple of 4096 residues, and residue
