@@ -90,6 +90,12 @@ def pass_plan(L, logL):
     return [base + (1 if i >= npass - extra else 0) for i in range(npass)]
 
 
+def has_xtab(logS, logT):
+    """Whether a pass of 2^logS x 2^logT takes its cross twiddles from a table (xtab_for in
+    ntt_engine.hip: S of 8..1024 points and a block of 2^8..2^20 residues)."""
+    return 3 <= logS <= 10 and 8 <= logS + logT <= 20
+
+
 def launch_table(L, logL, na, nb):
     """Per launch of one product (in launch order): rocprof kernel name, blocks, algorithmic bytes.
     Algorithmic bytes: residues are 8 B, k = 3 primes; forward passes cover both operands, the first
@@ -101,7 +107,7 @@ def launch_table(L, logL, na, nb):
     plan = pass_plan(L, logL)
     out, done = [], 0
     for i, s in enumerate(plan[:-1]):
-        kind = 3 if i == 0 else 0
+        kind = 3 if i == 0 else (6 if has_xtab(s, logL - done - s) else 0)
         byt = 8 * (na + nb) + 2 * K_PRIMES * 8 * Lw if i == 0 else 2 * K_PRIMES * 16 * Lw
         out.append((f"k_fast_pass<{s}, {kind}>", ntiles * K_PRIMES * 2, byt))
         done += s
@@ -111,6 +117,8 @@ def launch_table(L, logL, na, nb):
         logT = logL - done - plan[i]
         logC = min(logT, 12 - plan[i])
         kind = 5 if plan[i] >= 3 and logC + plan[i] == 12 else 1
+        if kind == 5 and has_xtab(plan[i], logT):
+            kind = 7
         out.append((f"k_fast_pass<{plan[i]}, {kind}>", ntiles * K_PRIMES, K_PRIMES * 16 * Lw))
     out.append(("k_garner_pack", None, K_PRIMES * 8 * Lw + 8 * (na + nb)))
     return out

@@ -45,6 +45,11 @@ struct Engine {
     uint8_t max_logs[NTT_MAX_LOG_LEN + 1] = {};
     /* most tiles one block of a forward / inverse pass works through */
     int tile_batch = 1;
+    /* cross-twiddle tables of passes whose S x T block has at most
+     * 2^NTT_XTAB_MAX_LOG residues, by (log2 S, log2 S*T), made on first use;
+     * use_xtab = 0 keeps every pass on the in-round Montgomery chains */
+    uint64_t *xtab[FT_MAX_LOGS + 1][NTT_XTAB_MAX_LOG + 1] = {};
+    int use_xtab = 1;
     unsigned long dispatches = 0;
     int timing = 0;
     /* timing marks: one event after each launch of the last multiply */
@@ -144,6 +149,12 @@ uint32_t pad_log_for(int logL);
 /* Workspace of a size class, and its transforms (+ Garner and carry unless
  * residues_in: then the rows hold residues on entry and on exit). */
 void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t &res_words, size_t &scratch);
+/* Cross-twiddle table pair (forward, inverse) of an S x T pass, or nullptr
+ * (tables off, block too large, or not yet made while a stream captures). */
+const uint64_t *xtab_for(int logS, int logT, hipStream_t st);
+/* Make the tables the passes of a length-2^logL transform use (before a
+ * graph capture, which cannot allocate). */
+void xtab_prepare(int logL);
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
                       uint32_t max_nout, int square, hipStream_t st, int residues_in = 0, uint32_t b_period = 0);
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);

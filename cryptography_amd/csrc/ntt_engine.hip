@@ -126,6 +126,9 @@ void engine_teardown()
     (void)hipFree(g.tw);
     (void)hipFree(g.small_tw);
     (void)hipFree(g.flag);
+    for (auto &row : g.xtab)
+        for (uint64_t *t : row)
+            if (t) (void)hipFree(t);
     g_ws.release();
     if (g.d_desc) (void)hipFree(g.d_desc);
     if (g.d_a) (void)hipFree(g.d_a);
@@ -137,13 +140,14 @@ void engine_teardown()
     }
     for (int i = 0; i < NTT_MAX_MARKS; i++) (void)hipEventDestroy(g.ev[i]);
     (void)hipStreamDestroy(g.stream);
-    int thr = g.threshold, tb = g.tile_batch;
+    int thr = g.threshold, tb = g.tile_batch, xt = g.use_xtab;
     uint8_t ml[NTT_MAX_LOG_LEN + 1];
     memcpy(ml, g.max_logs, sizeof ml);
     g.~Engine();
     new (&g) Engine();
     g.threshold = thr;
     g.tile_batch = tb;
+    g.use_xtab = xt;
     memcpy(g.max_logs, ml, sizeof ml);
 }
 
@@ -183,6 +187,43 @@ PassPlan plan_passes(int logL)
     int base = logL / pp.npass, extra = logL % pp.npass;
     for (int i = 0; i < pp.npass; i++) pp.logS[i] = base + (i >= pp.npass - extra ? 1 : 0);
     return pp;
+}
+
+const uint64_t *xtab_for(int logS, int logT, hipStream_t st)
+{
+    const int logN = logS + logT;
+    if (!g.use_xtab || logS < 3 || logS > FT_MAX_LOGS || logT < 0 || logN < 8 || logN > NTT_XTAB_MAX_LOG)
+        return nullptr;
+    uint64_t *&t = g.xtab[logS][logN];
+    if (t) return t;
+    /* made on the engine stream, which must not be recording a graph */
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone, cg = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+    if (hipStreamIsCapturing(g.stream, &cg) != hipSuccess || cg != hipStreamCaptureStatusNone) return nullptr;
+    uint64_t *p = nullptr;
+    if (hipMalloc((void **)&p, (size_t)2 * NTT_NPRIMES * ((size_t)8 << logN)) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;      /* the passes fall back to the chains */
+    }
+    /* complete before any stream's pass reads it */
+    if (launch_xtab_gen(g.tw, p, logS, logT, g.stream) != hipSuccess || hipStreamSynchronize(g.stream) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(p);
+        return nullptr;
+    }
+    t = p;
+    return t;
+}
+
+void xtab_prepare(int logL)
+{
+    if (logL <= 0) return;
+    const PassPlan pp = plan_passes(logL);
+    int done = 0;
+    for (int i = 0; i < pp.npass - 1; i++) {
+        (void)xtab_for((int)pp.logS[i], logL - done - (int)pp.logS[i], g.stream);
+        done += (int)pp.logS[i];
+    }
 }
 
 /* Residues of a class row: nprod * L rounded up to whole tiles. */
@@ -305,7 +346,10 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     for (int i = 0; i < pp.npass - 1; i++) {
         geom(i, done);
         pa.first = i == 0 && !residues_in;
-        HIP_TRY(launch_fast_pass(pp.logS[i], pa.first ? 3 : 0, pa, ntiles, fwd_ops, st));
+        int kind = pa.first ? 3 : 0;
+        pa.xtab = nullptr;
+        if (kind == 0 && (pa.xtab = xtab_for((int)pp.logS[i], (int)pa.logT, st))) kind = 6;
+        HIP_TRY(launch_fast_pass(pp.logS[i], kind, pa, ntiles, fwd_ops, st));
         HIP_TRY(mark("k_fast_pass<fwd>"));
         if (share) {   /* b of the first b_period products */
             PassArgs pb = pa;
@@ -314,12 +358,13 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
             pb.ntiles = (uint32_t)(b_period * (L / FT_TILE));
             pb.tpb = (uint32_t)std::min<size_t>(std::max<size_t>((size_t)pb.ntiles * NTT_NPRIMES / 4096, 1),
                                                 (size_t)g.tile_batch);
-            HIP_TRY(launch_fast_pass(pp.logS[i], pb.first ? 3 : 0, pb, pb.ntiles, 1, st));
+            HIP_TRY(launch_fast_pass(pp.logS[i], kind, pb, pb.ntiles, 1, st));
             HIP_TRY(mark("k_fast_pass<fwd b>"));
         }
         done += pp.logS[i];
     }
     geom(pp.npass - 1, done);
+    pa.xtab = nullptr;
     pa.first = pp.npass == 1 && !residues_in;
     pa.b_tiles = share ? (uint32_t)(b_period * (L / FT_TILE)) : 0;
     HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], pa.first ? 4 : 2, pa, ntiles, nops, st));
@@ -329,7 +374,15 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         done -= pp.logS[i];
         geom(i, done);
         pa.first = 0;
-        const int kind = (pp.logS[i] >= 3 && pa.logC + pp.logS[i] == 12) ? 5 : 1;
+        int kind = (pp.logS[i] >= 3 && pa.logC + pp.logS[i] == 12) ? 5 : 1;
+        pa.xtab = nullptr;
+        if (kind == 5) {
+            const uint64_t *xt = xtab_for((int)pp.logS[i], (int)pa.logT, st);
+            if (xt) {   /* the inverse table follows the forward one */
+                kind = 7;
+                pa.xtab = xt + ((size_t)NTT_NPRIMES << (pp.logS[i] + pa.logT));
+            }
+        }
         HIP_TRY(launch_fast_pass(pp.logS[i], kind, pa, ntiles, nops, st));
         HIP_TRY(mark("k_fast_pass<inv>"));
     }
@@ -827,6 +880,7 @@ int ntt_batch_graph_create(const ntt_mul_desc *descs, size_t n, ntt_batch_graph 
         ntt_batch_graph_destroy(gr);
         return rc;
     }
+    for (auto &c : gr->bp.classes) xtab_prepare(c.logL);   /* no allocation under capture */
     int saved_timing = g.timing;
     g.timing = 0;   /* no event nodes in the graph */
     hipError_t e = hipStreamBeginCapture(g.stream, hipStreamCaptureModeThreadLocal);
@@ -883,6 +937,13 @@ int ntt_engine_set_tile_batch(int tiles)
     return NTT_OK;
 }
 int ntt_engine_get_tile_batch(void) { return g.tile_batch; }
+int ntt_engine_set_xtab(int on)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    const int old = g.use_xtab;
+    g.use_xtab = on ? 1 : 0;
+    return old;
+}
 void ntt_engine_set_threshold(int limbs) { g.threshold = limbs; }
 unsigned long ntt_engine_dispatch_count(void) { return g.dispatches; }
 void ntt_engine_set_timing(int on) { g.timing = on; }

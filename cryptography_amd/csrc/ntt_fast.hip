@@ -14,6 +14,9 @@
  *     LDS as Shoup pairs; the three smallest stages use compile-time twiddles;
  *   - the cross twiddles are applied inside the last forward (first inverse)
  *     radix round as a Montgomery chain, four table look-ups per 8 points;
+ *     passes whose S x T block has at most 2^20 residues (the inner pass of
+ *     a three-pass transform, the inverse of a two-pass one) instead read one
+ *     precomputed Montgomery-form twiddle per residue (kinds 6 and 7);
  *   - the last forward pass of both operands, the pointwise product (with the
  *     1/L scale) and the first inverse pass are one kernel (kind 2): both
  *     spectra of a tile are formed, multiplied and transformed back without
@@ -201,6 +204,31 @@ __device__ __forceinline__ uint64_t tw_mont(const TwPair *tw, uint32_t E)
     /* lo.w * 2^64 mod p = -lo.wp * p mod 2^64 (the Shoup remainder) */
     uint64_t lo_m = (uint64_t)0 - lo.wp * Prime<PI>::P;
     return shoup_lazy<PI>(lo_m, hi.w, hi.wp);
+}
+
+/* Cross-twiddle table entry e = m*T + r of an S x T pass (N = S*T):
+ * w_N^(r*bitrev_S(m)) in Montgomery form, w_N^-(r*bitrev_S(m)) for INV,
+ * reduced to [0, p). */
+template <int PI>
+__device__ __forceinline__ uint64_t xtab_entry(const TwPair *tw, uint32_t logS, uint32_t logT, uint32_t e, bool inv)
+{
+    const uint32_t m = e >> logT, r = e & ((1u << logT) - 1);
+    uint32_t E = (r * brev_n(m, logS)) << (NTT_ROOT_LOG - logS - logT);
+    if (inv) E = 0u - E;
+    return reduce1<PI>(tw_mont<PI>(tw + (size_t)PI * 2 * NTT_TW_SIZE, E & ((1u << NTT_ROOT_LOG) - 1)));
+}
+
+/* grid (N / 256, primes, 2 directions) */
+__global__ void __launch_bounds__(256) k_xtab_gen(const TwPair *tw, uint64_t *out, uint32_t logS, uint32_t logT)
+{
+    const uint32_t N = 1u << (logS + logT);
+    const uint32_t e = blockIdx.x * 256 + threadIdx.x, pi = blockIdx.y;
+    const bool inv = blockIdx.z != 0;
+    if (e >= N) return;
+    const uint64_t v = pi == 0   ? xtab_entry<0>(tw, logS, logT, e, inv)
+                       : pi == 1 ? xtab_entry<1>(tw, logS, logT, e, inv)
+                                 : xtab_entry<2>(tw, logS, logT, e, inv);
+    out[((size_t)blockIdx.z * NTT_NPRIMES + pi) * N + e] = v;
 }
 
 /* x[i] *= w^(EA + bitrev_RL(i) * EB) for i < 2^RL (exponents mod 2^30);
@@ -514,7 +542,12 @@ __device__ __forceinline__ void load_tile(const PassArgs &pa, const uint64_t *x,
     }
 }
 
-template <int PI, int LOGS, bool FIRST>
+/* XTAB: the cross twiddles come from pa.xtab and multiply the tile as it is
+ * stored (element q = tid + k*FTPB sits in row (q >> logC) mod S, column
+ * col0 + (q mod C)): one Montgomery product per residue instead of the
+ * in-round chain's two, for passes whose S x T block is at most
+ * 2^NTT_XTAB_MAX_LOG residues (its table stays in the L2 caches). */
+template <int PI, int LOGS, bool FIRST, bool XTAB>
 __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds, TwPair *twf)
 {
     const FGeom g{pa.logT, pa.logC, pa.logG};
@@ -527,7 +560,7 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
     stage_twiddles<PI, LOGS, 1>(pa.small_tw, twf, nullptr);
     if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, t, lds);
     xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
-    if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, false>(xc, g.logC);
+    if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false>(xc, g.logC);
     if constexpr (!FIRST) WAIT_VMCNT(0);
     lds_sync();
     for (;;) {
@@ -536,20 +569,34 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
             fill_first_tile<LOGS>(pa, op, g, t, lds);
             lds_sync();
         }
-        dif_all<PI, LOGS, LOGS - 1, true>(lds, twf, g.logC, xc);   /* cross twiddles in the last round */
+        dif_all<PI, LOGS, LOGS - 1, !XTAB>(lds, twf, g.logC, xc);   /* chain: cross twiddles in the last round */
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         {
             size_t base, step;
             tile_span<LOGS>(pa, g, t, tid, base, step);
+            if constexpr (XTAB) {
+                const uint64_t *xt = pa.xtab + ((size_t)PI << (LOGS + g.logT));
+                uint64_t w[FT_PER];
 #pragma unroll
-            for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
+                for (int k = 0; k < FT_PER; k++) {
+                    const uint32_t q = tid + k * FTPB;
+                    const uint32_t m = (q >> g.logC) & ((1u << LOGS) - 1), c = q & ((1u << g.logC) - 1);
+                    w[k] = xt[((size_t)m << g.logT) + xc.col0 + c];
+                }
+#pragma unroll
+                for (int k = 0; k < FT_PER; k++)
+                    x[base + k * step] = mont_mul_lazy<PI>(lds[LP(tid + k * FTPB)], w[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
+            }
         }
         if (++t >= tend) break;
         lds_sync();
         if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, t, lds);
         xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
-        if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, false>(xc, g.logC);
+        if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false>(xc, g.logC);
         if constexpr (!FIRST) {
             WAIT_VMCNT(0);
             lds_sync();
@@ -563,9 +610,10 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
  * cross twiddles are applied as a chain right after the load and the DIT
  * rounds carry no twiddle code (fewer live registers).  Otherwise they are
  * applied in the first DIT round. */
-template <int PI, int LOGS, bool LOADCHAIN>
+template <int PI, int LOGS, bool LOADCHAIN, bool XTAB>
 __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair *twi)
 {
+    static_assert(LOADCHAIN || !XTAB, "table twiddles are applied at the load");
     const FGeom g{pa.logT, pa.logC, pa.logG};
     uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
@@ -588,7 +636,17 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
             load_tile<LOGS, false>(pa, x, 0, g, t, v);
             asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
             const uint32_t r = xc.col0 + (tid & ((1u << g.logC) - 1)), m0 = tid >> g.logC;
-            chain_twiddle<PI, 3>(v, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift), 0u - (r << xc.shift));
+            if constexpr (XTAB) {
+                /* rows m0 + k * 2^(LOGS-3) of column r: one table entry each */
+                const uint64_t *xt = pa.xtab + ((size_t)PI << (LOGS + g.logT)) + ((size_t)m0 << g.logT) + r;
+                uint64_t w[FT_PER];
+#pragma unroll
+                for (int k = 0; k < FT_PER; k++) w[k] = xt[(size_t)k << (LOGS - 3 + g.logT)];
+#pragma unroll
+                for (int k = 0; k < FT_PER; k++) v[k] = mont_mul_lazy<PI>(v[k], w[k]);
+            } else {
+                chain_twiddle<PI, 3>(v, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift), 0u - (r << xc.shift));
+            }
 #pragma unroll
             for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
             lds_sync();
@@ -730,17 +788,20 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 
 /* kind 0: forward pass, 1: inverse pass, 2: fused middle pass,
  * 3: first forward pass (reads the operands), 4: single fused pass,
- * 5: inverse pass with the cross twiddles applied at load (C * S = 4096). */
+ * 5: inverse pass with the cross twiddles applied at load (C * S = 4096),
+ * 6: kind 0 and 7: kind 5 with the cross twiddles from pa.xtab. */
 template <int PI, int LOGS, int KIND>
 __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, int op,
                                           uint64_t *lds, TwPair *twf, TwPair *twi)
 {
     /* each routine stages its stage twiddles itself, after starting its
      * first tile's transfer */
-    if constexpr (KIND == 0) fwd_tiles<PI, LOGS, false>(pa, xa, op, lds, twf);
-    else if constexpr (KIND == 3) fwd_tiles<PI, LOGS, true>(pa, xa, op, lds, twf);
-    else if constexpr (KIND == 1) inv_tiles<PI, LOGS, false>(pa, xa, lds, twi);
-    else if constexpr (KIND == 5) inv_tiles<PI, LOGS, true>(pa, xa, lds, twi);
+    if constexpr (KIND == 0) fwd_tiles<PI, LOGS, false, false>(pa, xa, op, lds, twf);
+    else if constexpr (KIND == 6) fwd_tiles<PI, LOGS, false, true>(pa, xa, op, lds, twf);
+    else if constexpr (KIND == 3) fwd_tiles<PI, LOGS, true, false>(pa, xa, op, lds, twf);
+    else if constexpr (KIND == 1) inv_tiles<PI, LOGS, false, false>(pa, xa, lds, twi);
+    else if constexpr (KIND == 5) inv_tiles<PI, LOGS, true, false>(pa, xa, lds, twi);
+    else if constexpr (KIND == 7) inv_tiles<PI, LOGS, true, true>(pa, xa, lds, twi);
     else if constexpr (KIND == 2) mid_tile<PI, LOGS, false>(pa, xa, xb, lds, twf, twi);
     else mid_tile<PI, LOGS, true>(pa, xa, xb, lds, twf, twi);
 }
@@ -768,13 +829,16 @@ __global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
 hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntiles, int nops, hipStream_t st)
 {
     /* forward and inverse passes: tpb tiles per block; fused passes: one */
-    const bool looped = kind == 0 || kind == 1 || kind == 3 || kind == 5;
+    const bool looped = kind == 0 || kind == 1 || kind == 3 || kind == 5 || kind == 6 || kind == 7;
     if (looped && (pa.tpb < 1 || pa.ntiles != ntiles)) return hipErrorInvalidValue;
-    if (kind == 5 && (logS < 3 || pa.logC + logS != 12)) return hipErrorInvalidValue;
+    if ((kind == 5 || kind == 7) && (logS < 3 || pa.logC + logS != 12)) return hipErrorInvalidValue;
     /* tile transfers move column pairs (C >= 2) or row pairs (T = 1) */
-    if ((kind == 0 || kind == 1) && pa.logC == 0 && pa.logT != 0) return hipErrorInvalidValue;
+    if ((kind == 0 || kind == 1 || kind == 6) && pa.logC == 0 && pa.logT != 0) return hipErrorInvalidValue;
+    /* table twiddles: a table of this pass's S x T block */
+    if ((kind == 6 || kind == 7) && (!pa.xtab || logS + (int)pa.logT > NTT_XTAB_MAX_LOG)) return hipErrorInvalidValue;
     const size_t nblk = looped ? (ntiles + pa.tpb - 1) / pa.tpb : ntiles;
-    dim3 grid((unsigned)nblk, NTT_NPRIMES, (kind == 0 || kind == 3) ? nops : 1);
+    const bool fwd = kind == 0 || kind == 3 || kind == 6;
+    dim3 grid((unsigned)nblk, NTT_NPRIMES, fwd ? nops : 1);
     dim3 block(FTPB);
 #define L1(S)                                                                                  \
     case S:                                                                                    \
@@ -796,6 +860,8 @@ hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntile
         case 3: hipLaunchKernelGGL((k_fast_pass<S, 3>), grid, block, 0, st, pa); break;        \
         case 4: hipLaunchKernelGGL((k_fast_pass<S, 4>), grid, block, 0, st, pa); break;        \
         case 5: hipLaunchKernelGGL((k_fast_pass<S, 5>), grid, block, 0, st, pa); break;        \
+        case 6: hipLaunchKernelGGL((k_fast_pass<S, 6>), grid, block, 0, st, pa); break;        \
+        case 7: hipLaunchKernelGGL((k_fast_pass<S, 7>), grid, block, 0, st, pa); break;        \
         default: return hipErrorInvalidValue;                                                  \
         }                                                                                      \
         break;
@@ -806,6 +872,15 @@ hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntile
     }
 #undef L1
 #undef L5
+    return hipGetLastError();
+}
+
+hipError_t launch_xtab_gen(const TwPair *tw, uint64_t *out, int logS, int logT, hipStream_t st)
+{
+    const int logN = logS + logT;
+    if (logS < 1 || logT < 0 || logN > NTT_XTAB_MAX_LOG || logN < 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_xtab_gen, dim3(1u << (logN - 8), NTT_NPRIMES, 2), dim3(256), 0, st, tw, out, (uint32_t)logS,
+                       (uint32_t)logT);
     return hipGetLastError();
 }
 

@@ -58,12 +58,20 @@ struct PassArgs {
     uint32_t b_tiles;           /* fused pass: b spectra repeat, tile t's is at
                                    t mod b_tiles (0: every product has its own) */
     uint64_t scale[3], scale_p[3];   /* 2^64 / L mod p and Shoup partners (middle pass) */
+    const uint64_t *xtab;       /* kinds 6 and 7: this pass's cross twiddles, one
+                                   entry per residue of an S x T block, per prime */
 };
 
 #define FT_TILE 4096            /* residues per transform tile */
 #define FT_MAX_LOGS 10          /* largest sub-transform of one pass */
 hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntiles, int nops, hipStream_t st);
 hipError_t launch_small_twiddles(const TwPair *tw, TwPair *small, hipStream_t st);
+/* Cross-twiddle tables of a pass with S = 2^logS rows and T = 2^logT columns
+ * (N = S*T <= 2^NTT_XTAB_MAX_LOG): out[dir][prime][m*T + r] =
+ * w_N^(+-r*bitrev_S(m)) * 2^64 mod p (Montgomery form, in [0, p)); dir 0
+ * forward, 1 inverse. */
+#define NTT_XTAB_MAX_LOG 20
+hipError_t launch_xtab_gen(const TwPair *tw, uint64_t *out, int logS, int logT, hipStream_t st);
 
 struct GarnerConsts {
     uint64_t inv01, inv01p;      /* p0^-1 mod p1 */

@@ -306,6 +306,8 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
     }
     if (!rc) rc = ws_reserve(t->ws, res_words, scratch);
     if (!rc && use_graph == 1) {
+        for (int h = 1; h <= H; h++)      /* tables first: nothing allocates under capture */
+            for (auto &c : t->levels[h].classes) xtab_prepare(c.logL);
         if (hipStreamSynchronize(g.stream) != hipSuccess) rc = fail(NTT_ERR_HIP, "stream sync");
         int saved = g.timing;
         g.timing = 0;

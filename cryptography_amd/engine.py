@@ -340,6 +340,15 @@ def set_tile_batch(tiles):
     return old
 
 
+def set_xtab(on):
+    """Cross twiddles of passes with at most 2^20-residue column blocks from
+    per-block tables (True) or from in-pass Montgomery chains (False);
+    returns the old setting."""
+    L = lib()
+    L.ntt_engine_set_xtab.argtypes = [ctypes.c_int]
+    return bool(L.ntt_engine_set_xtab(1 if on else 0))
+
+
 def get_tile_batch():
     """Current tiles per block of the long forward / inverse passes."""
     return lib().ntt_engine_get_tile_batch()

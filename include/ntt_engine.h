@@ -101,6 +101,11 @@ int ntt_engine_get_pass_cap(int log_len);
  * only, small ones use 1). */
 int ntt_engine_set_tile_batch(int tiles);
 int ntt_engine_get_tile_batch(void);
+/* Cross twiddles of passes whose column block has at most 2^20 residues:
+ * from precomputed per-block tables (1, the default; one Montgomery product
+ * per residue) or from Montgomery chains inside the pass (0).  Both give the
+ * same products.  Returns the previous setting. */
+int ntt_engine_set_xtab(int on);
 
 /* Autotuner: times the schoolbook/NTT crossover on batches of equal-size
  * products, every pass split for lengths 2^min_log..2^max_log (11..28) and,

@@ -142,6 +142,28 @@ def test_three_pass_transform(na, nb):
     assert np.array_equal(E.sqr(a), O.gmp_mul(a, a))
 
 
+@pytest.mark.parametrize("na,nb", [(1 << 18, (1 << 18) + 5), (3 << 17, 1 << 16), (5000, 3000), (1 << 22, 1 << 22)])
+def test_cross_twiddle_tables_match_chains(na, nb):
+    """Cross twiddles from per-block tables (forward column passes at store,
+    inverse passes at load) against the in-pass Montgomery chains: the same
+    products, both equal to GMP.  Lengths 2^21 (three passes of 2^7: the
+    middle block of 2^14 has tables), 2^15 (two passes: the inverse of the
+    first pass, a 2^15 block, has them) and 2^25 (three passes, 2^17 block)."""
+    E.init()
+    a = O.splitmix(301 + na, na)
+    b = O.splitmix(303 + nb, nb)
+    want_ab, want_aa = O.gmp_mul(a, b), O.gmp_mul(a, a)
+    old = E.engine.set_xtab(False)
+    try:
+        chain_ab, chain_aa = E.mul(a, b), E.sqr(a)
+        E.engine.set_xtab(True)
+        tab_ab, tab_aa = E.mul(a, b), E.sqr(a)
+    finally:
+        E.engine.set_xtab(old)
+    assert np.array_equal(chain_ab, want_ab) and np.array_equal(chain_aa, want_aa)
+    assert np.array_equal(tab_ab, want_ab) and np.array_equal(tab_aa, want_aa)
+
+
 @pytest.mark.parametrize("tiles", [1, 2, 3, 4, 8])
 def test_tile_batch_parity(tiles):
     """Forward / inverse passes with several tiles per block (the next tile's
