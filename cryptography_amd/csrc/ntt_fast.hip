@@ -87,6 +87,20 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n)
     return x * q + (x < rem ? x : rem) + i;
 }
 
+/* Tile order of the table-twiddle passes: tile t of the launch covers column
+ * tile ct of group (gb*8 + gi) for t = (gb*colTiles + ct)*8 + gi, so the
+ * tiles one XCD works on at a time share their twiddle columns across 8
+ * groups (the table's slice stays in L2), and column neighbours are 8 tiles
+ * apart (they still share their 128-byte lines).  Needs a multiple of 8
+ * groups; otherwise the order is unchanged. */
+__device__ __forceinline__ uint32_t xtab_tile(uint32_t t, uint32_t logCT, uint32_t ntiles)
+{
+    if (((ntiles >> logCT) & 7) != 0 || (ntiles & ((1u << logCT) - 1)) != 0) return t;
+    const uint32_t gi = t & 7, rest = t >> 3;
+    const uint32_t ct = rest & ((1u << logCT) - 1), gb = rest >> logCT;
+    return (((gb << 3) | gi) << logCT) | ct;
+}
+
 __device__ __forceinline__ uint32_t brev_n(uint32_t x, int bits)
 {
     return bits ? (__brev(x) >> (32 - bits)) : 0;
@@ -554,19 +568,20 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
     uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
     const uint32_t colTiles = g.logT - g.logC;
+    uint32_t tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;   /* the tile worked on */
     XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0};
     /* the first tile and the cross-twiddle look-ups are in flight while
      * the stage twiddles are written */
     stage_twiddles<PI, LOGS, 1>(pa.small_tw, twf, nullptr);
-    if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, t, lds);
-    xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+    if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, tt, lds);
+    xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
     if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false>(xc, g.logC);
     if constexpr (!FIRST) WAIT_VMCNT(0);
     lds_sync();
     for (;;) {
         uint32_t tid;
         if constexpr (FIRST) {
-            fill_first_tile<LOGS>(pa, op, g, t, lds);
+            fill_first_tile<LOGS>(pa, op, g, tt, lds);
             lds_sync();
         }
         dif_all<PI, LOGS, LOGS - 1, !XTAB>(lds, twf, g.logC, xc);   /* chain: cross twiddles in the last round */
@@ -574,7 +589,7 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
         asm volatile("" : "+v"(tid));
         {
             size_t base, step;
-            tile_span<LOGS>(pa, g, t, tid, base, step);
+            tile_span<LOGS>(pa, g, tt, tid, base, step);
             if constexpr (XTAB) {
                 const uint64_t *xt = pa.xtab + ((size_t)PI << (LOGS + g.logT));
                 uint64_t w[FT_PER];
@@ -593,9 +608,10 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
             }
         }
         if (++t >= tend) break;
+        tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;
         lds_sync();
-        if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, t, lds);
-        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+        if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, tt, lds);
+        xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
         if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false>(xc, g.logC);
         if constexpr (!FIRST) {
             WAIT_VMCNT(0);
@@ -618,13 +634,14 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
     uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
     const uint32_t colTiles = g.logT - g.logC;
+    uint32_t tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;   /* the tile worked on */
     XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0};
     /* without the load chain the tile goes straight into LDS, in flight
      * while the stage twiddles are written */
     stage_twiddles<PI, LOGS, 2>(pa.small_tw, nullptr, twi);
-    xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+    xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
     if constexpr (!LOADCHAIN) {
-        tile_dma<LOGS>(pa, x, g, t, lds);
+        tile_dma<LOGS>(pa, x, g, tt, lds);
         if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, true>(xc, g.logC);
         WAIT_VMCNT(0);
     }
@@ -633,7 +650,7 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
         uint32_t tid = threadIdx.x;
         if constexpr (LOADCHAIN) {
             uint64_t v[FT_PER];
-            load_tile<LOGS, false>(pa, x, 0, g, t, v);
+            load_tile<LOGS, false>(pa, x, 0, g, tt, v);
             asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
             const uint32_t r = xc.col0 + (tid & ((1u << g.logC) - 1)), m0 = tid >> g.logC;
             if constexpr (XTAB) {
@@ -657,15 +674,16 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
         asm volatile("" : "+v"(tid));
         {
             size_t base, step;
-            tile_span<LOGS>(pa, g, t, tid, base, step);
+            tile_span<LOGS>(pa, g, tt, tid, base, step);
 #pragma unroll
             for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
         }
         if (++t >= tend) break;
+        tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;
         lds_sync();
-        xc.col0 = (t & ((1u << colTiles) - 1)) << g.logC;
+        xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
         if constexpr (!LOADCHAIN) {
-            tile_dma<LOGS>(pa, x, g, t, lds);
+            tile_dma<LOGS>(pa, x, g, tt, lds);
             if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, true>(xc, g.logC);
             WAIT_VMCNT(0);
             lds_sync();
