@@ -748,18 +748,18 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     }
     lds_sync();
     dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
-    /* a's smallest round, kept in registers */
+    /* a's smallest round, kept in registers: its groups leave LDS first */
 #pragma unroll
     for (int n = 0; n < NG; n++) {
         const uint32_t gid = threadIdx.x + n * FTPB;
         const uint32_t base = ((gid >> LNB) << LOGS) + ((gid & ((1u << LNB) - 1)) << RLL);
 #pragma unroll
         for (int i = 0; i < RR; i++) ra[n * RR + i] = lds[LP(base + i)];
-        const_dif<PI, RLL, 0>(ra + n * RR);
     }
-    /* b's tile into the same LDS once a's is in registers: a second buffer
+    /* b's tile into the same LDS once a's is in registers (a second buffer
      * would let it arrive during a's rounds, but at 80 KB per block two
-     * blocks share a CU instead of three, which was measured slower */
+     * blocks share a CU instead of three, which was measured slower); the
+     * transfer is in flight during a's register round */
     if (!pa.square) {
         lds_sync();
         if constexpr (FIRST) {
@@ -770,8 +770,19 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
             }
         } else {
             tile_dma<LOGS>(pa, xb, g, pa.b_tiles ? t % pa.b_tiles : t, lds);
-            WAIT_VMCNT(0);
         }
+    }
+    /* a's round between the transfer's issue and its wait: the empty asm
+     * statements pin it there (the scheduler would otherwise sink it to the
+     * pointwise product, after b's rounds) */
+#pragma unroll
+    for (int i = 0; i < FT_PER; i++) asm volatile("" : "+v"(ra[i]));
+#pragma unroll
+    for (int n = 0; n < NG; n++) const_dif<PI, RLL, 0>(ra + n * RR);
+#pragma unroll
+    for (int i = 0; i < FT_PER; i++) asm volatile("" : "+v"(ra[i]));
+    if (!pa.square) {
+        if constexpr (!FIRST) WAIT_VMCNT(0);
         lds_sync();
         dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
     }
