@@ -201,12 +201,19 @@ const uint64_t *xtab_for(int logS, int logT, hipStream_t st)
     if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
     if (hipStreamIsCapturing(g.stream, &cg) != hipSuccess || cg != hipStreamCaptureStatusNone) return nullptr;
     uint64_t *p = nullptr;
-    if (hipMalloc((void **)&p, (size_t)2 * NTT_NPRIMES * ((size_t)8 << logN)) != hipSuccess) {
+    if (hipMalloc((void **)&p, (size_t)NTT_XTAB_DIRS * NTT_NPRIMES * ((size_t)8 << logN)) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;      /* the passes fall back to the chains */
     }
+    XtabScale sc{};
+    for (int i = 0; i < NTT_NPRIMES; i++) {   /* 2^64 / N mod p */
+        const uint64_t q = prime(i), r = (uint64_t)(((u128)1 << 64) % q);
+        sc.s[i] = h_mulmod(r, h_inv(((uint64_t)1 << logN) % q, q), q);
+        sc.sp[i] = h_shoup(sc.s[i], q);
+    }
     /* complete before any stream's pass reads it */
-    if (launch_xtab_gen(g.tw, p, logS, logT, g.stream) != hipSuccess || hipStreamSynchronize(g.stream) != hipSuccess) {
+    if (launch_xtab_gen(g.tw, p, logS, logT, sc, g.stream) != hipSuccess ||
+        hipStreamSynchronize(g.stream) != hipSuccess) {
         (void)hipGetLastError();
         (void)hipFree(p);
         return nullptr;
@@ -324,6 +331,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         uint64_t r = (uint64_t)(((u128)1 << 64) % p);
         pa.scale[i] = h_mulmod(r, h_inv(L % p, p), p);
         pa.scale_p[i] = h_shoup(pa.scale[i], p);
+        pa.scale_m[i] = h_mulmod(pa.scale[i], r, p);
     }
     /* geometry of pass i: S = 2^logS columns of stride T inside blocks of S*T */
     auto geom = [&](int i, uint32_t done) {
@@ -363,12 +371,19 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         }
         done += pp.logS[i];
     }
+    /* the last inverse pass (pass 0's) applies the 1/L scale when it loads
+     * its tile with the cross twiddles (kinds 5 and 7: its chain's first
+     * twiddle or its table carries the scale); otherwise the middle pass does */
+    geom(0, 0);
+    const bool scale_late = pp.npass >= 2 && pp.logS[0] >= 3 && pa.logC + pp.logS[0] == 12;
     geom(pp.npass - 1, done);
     pa.xtab = nullptr;
     pa.first = pp.npass == 1 && !residues_in;
     pa.b_tiles = share ? (uint32_t)(b_period * (L / FT_TILE)) : 0;
+    pa.scale_late = scale_late;
     HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], pa.first ? 4 : 2, pa, ntiles, nops, st));
     pa.b_tiles = 0;
+    pa.scale_late = 0;
     HIP_TRY(mark("k_fast_pass<mid>"));
     for (int i = pp.npass - 2; i >= 0; i--) {
         done -= pp.logS[i];
@@ -376,14 +391,17 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         pa.first = 0;
         int kind = (pp.logS[i] >= 3 && pa.logC + pp.logS[i] == 12) ? 5 : 1;
         pa.xtab = nullptr;
+        pa.iscale = i == 0 && scale_late;
         if (kind == 5) {
             const uint64_t *xt = xtab_for((int)pp.logS[i], (int)pa.logT, st);
-            if (xt) {   /* the inverse table follows the forward one */
+            if (xt) {   /* inverse tables follow the forward one; pass 0's block is
+                           the whole transform, so its scaled table is the 1/L one */
                 kind = 7;
-                pa.xtab = xt + ((size_t)NTT_NPRIMES << (pp.logS[i] + pa.logT));
+                pa.xtab = xt + ((size_t)(pa.iscale ? 2 : 1) * NTT_NPRIMES << (pp.logS[i] + pa.logT));
             }
         }
         HIP_TRY(launch_fast_pass(pp.logS[i], kind, pa, ntiles, nops, st));
+        pa.iscale = 0;
         HIP_TRY(mark("k_fast_pass<inv>"));
     }
     g.ntt_mark = g.n_marks - 1;

@@ -232,16 +232,21 @@ __device__ __forceinline__ uint64_t xtab_entry(const TwPair *tw, uint32_t logS, 
     return reduce1<PI>(tw_mont<PI>(tw + (size_t)PI * 2 * NTT_TW_SIZE, E & ((1u << NTT_ROOT_LOG) - 1)));
 }
 
-/* grid (N / 256, primes, 2 directions) */
-__global__ void __launch_bounds__(256) k_xtab_gen(const TwPair *tw, uint64_t *out, uint32_t logS, uint32_t logT)
+/* grid (N / 256, primes, NTT_XTAB_DIRS) */
+__global__ void __launch_bounds__(256) k_xtab_gen(const TwPair *tw, uint64_t *out, uint32_t logS, uint32_t logT,
+                                                  XtabScale sc)
 {
     const uint32_t N = 1u << (logS + logT);
     const uint32_t e = blockIdx.x * 256 + threadIdx.x, pi = blockIdx.y;
     const bool inv = blockIdx.z != 0;
     if (e >= N) return;
-    const uint64_t v = pi == 0   ? xtab_entry<0>(tw, logS, logT, e, inv)
-                       : pi == 1 ? xtab_entry<1>(tw, logS, logT, e, inv)
-                                 : xtab_entry<2>(tw, logS, logT, e, inv);
+    uint64_t v = pi == 0   ? xtab_entry<0>(tw, logS, logT, e, inv)
+                 : pi == 1 ? xtab_entry<1>(tw, logS, logT, e, inv)
+                           : xtab_entry<2>(tw, logS, logT, e, inv);
+    if (blockIdx.z == 2)
+        v = pi == 0   ? reduce1<0>(shoup_lazy<0>(v, sc.s[0], sc.sp[0]))
+            : pi == 1 ? reduce1<1>(shoup_lazy<1>(v, sc.s[1], sc.sp[1]))
+                      : reduce1<2>(shoup_lazy<2>(v, sc.s[2], sc.sp[2]));
     out[((size_t)blockIdx.z * NTT_NPRIMES + pi) * N + e] = v;
 }
 
@@ -662,7 +667,10 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
 #pragma unroll
                 for (int k = 0; k < FT_PER; k++) v[k] = mont_mul_lazy<PI>(v[k], w[k]);
             } else {
-                chain_twiddle<PI, 3>(v, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift), 0u - (r << xc.shift));
+                constexpr uint32_t MASK = (1u << NTT_ROOT_LOG) - 1;
+                uint64_t ta = tw_mont<PI>(xc.tw, (0u - ((r * brev_n(m0, LOGS)) << xc.shift)) & MASK);
+                if (pa.iscale) ta = mont_mul_lazy<PI>(ta, pa.scale_m[PI]);   /* the 1/L rides on the chain */
+                chain_apply<PI, 3>(v, ta, tw_mont<PI>(xc.tw, (0u - (r << xc.shift)) & MASK));
             }
 #pragma unroll
             for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
@@ -800,8 +808,13 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
             for (int i = 0; i < RR; i++) x[i] = lds[LP(base + i)];
             const_dif<PI, RLL, 0>(x);
         }
+        if (pa.scale_late) {    /* the last inverse pass applies the 1/L scale */
 #pragma unroll
-        for (int i = 0; i < RR; i++) x[i] = shoup_lazy<PI>(mont_mul_lazy<PI>(ra[n * RR + i], x[i]), s, sp);
+            for (int i = 0; i < RR; i++) x[i] = mont_mul_lazy<PI>(ra[n * RR + i], x[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < RR; i++) x[i] = shoup_lazy<PI>(mont_mul_lazy<PI>(ra[n * RR + i], x[i]), s, sp);
+        }
         const_dit<PI, RLL, RLL - 1>(x);
 #pragma unroll
         for (int i = 0; i < RR; i++) lds[LP(base + i)] = x[i];
@@ -904,12 +917,12 @@ hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntile
     return hipGetLastError();
 }
 
-hipError_t launch_xtab_gen(const TwPair *tw, uint64_t *out, int logS, int logT, hipStream_t st)
+hipError_t launch_xtab_gen(const TwPair *tw, uint64_t *out, int logS, int logT, const XtabScale &sc, hipStream_t st)
 {
     const int logN = logS + logT;
     if (logS < 1 || logT < 0 || logN > NTT_XTAB_MAX_LOG || logN < 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_xtab_gen, dim3(1u << (logN - 8), NTT_NPRIMES, 2), dim3(256), 0, st, tw, out, (uint32_t)logS,
-                       (uint32_t)logT);
+    hipLaunchKernelGGL(k_xtab_gen, dim3(1u << (logN - 8), NTT_NPRIMES, NTT_XTAB_DIRS), dim3(256), 0, st, tw, out,
+                       (uint32_t)logS, (uint32_t)logT, sc);
     return hipGetLastError();
 }
 

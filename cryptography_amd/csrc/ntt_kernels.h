@@ -58,6 +58,10 @@ struct PassArgs {
     uint32_t b_tiles;           /* fused pass: b spectra repeat, tile t's is at
                                    t mod b_tiles (0: every product has its own) */
     uint64_t scale[3], scale_p[3];   /* 2^64 / L mod p and Shoup partners (middle pass) */
+    uint64_t scale_m[3];        /* the same times 2^64 mod p (Montgomery form) */
+    int scale_late;             /* middle pass: leave the 1/L scale to the last inverse pass */
+    int iscale;                 /* kind 5: multiply by 1/L (the chain's first twiddle carries it);
+                                   kind 7 reads a table that includes it */
     const uint64_t *xtab;       /* kinds 6 and 7: this pass's cross twiddles, one
                                    entry per residue of an S x T block, per prime */
 };
@@ -69,9 +73,14 @@ hipError_t launch_small_twiddles(const TwPair *tw, TwPair *small, hipStream_t st
 /* Cross-twiddle tables of a pass with S = 2^logS rows and T = 2^logT columns
  * (N = S*T <= 2^NTT_XTAB_MAX_LOG): out[dir][prime][m*T + r] =
  * w_N^(+-r*bitrev_S(m)) * 2^64 mod p (Montgomery form, in [0, p)); dir 0
- * forward, 1 inverse. */
+ * forward, 1 inverse, 2 inverse times scale[prime] (2^64 / N mod p: the
+ * 1/N of a whole length-N transform folded into its last inverse pass). */
 #define NTT_XTAB_MAX_LOG 20
-hipError_t launch_xtab_gen(const TwPair *tw, uint64_t *out, int logS, int logT, hipStream_t st);
+#define NTT_XTAB_DIRS 3
+struct XtabScale {
+    uint64_t s[3], sp[3];       /* 2^64 / N mod p and Shoup partners */
+};
+hipError_t launch_xtab_gen(const TwPair *tw, uint64_t *out, int logS, int logT, const XtabScale &sc, hipStream_t st);
 
 struct GarnerConsts {
     uint64_t inv01, inv01p;      /* p0^-1 mod p1 */
