@@ -122,6 +122,7 @@ struct BatchPlan {
         /* products i and i + b_period have the same b (pointer and length):
          * 1 when all share one, count when none repeats */
         uint32_t b_period = 0;
+        uint32_t max_nab = 0;   /* longest operand (limbs) */
     };
     std::vector<MulDesc, uninit_alloc<MulDesc>> sorted;   /* descriptors grouped by class */
     std::vector<Class> classes;
@@ -156,7 +157,8 @@ const uint64_t *xtab_for(int logS, int logT, hipStream_t st);
  * graph capture, which cannot allocate). */
 void xtab_prepare(int logL);
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
-                      uint32_t max_nout, int square, hipStream_t st, int residues_in = 0, uint32_t b_period = 0);
+                      uint32_t max_nout, int square, hipStream_t st, int residues_in = 0, uint32_t b_period = 0,
+                      uint32_t max_nab = 0);
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);
 
 /* Host threads for planning work (descriptor sorting, tree construction):

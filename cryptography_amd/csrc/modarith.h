@@ -76,6 +76,18 @@ __host__ __device__ __forceinline__ uint64_t shoup_lazy(uint64_t a, uint64_t w, 
     return a * w - q * Prime<PI>::P;
 }
 
+/* Shoup product a*w mod p for a < 2^32; result in [0, 2p).  q = hi64(a*wp)
+ * exactly: a*wp = a*wp1*2^32 + a*wp0, and with t = a*wp1 + hi32(a*wp0) (an
+ * integer below 2^64) floor(a*wp / 2^64) = floor(t / 2^32), since the
+ * dropped part is below 1.  Five 32-bit multiplies instead of nine. */
+template <int PI>
+__host__ __device__ __forceinline__ uint64_t shoup_small(uint32_t a, uint64_t w, uint64_t wp)
+{
+    const uint64_t t = (uint64_t)a * (uint32_t)(wp >> 32) + (uint32_t)(((uint64_t)a * (uint32_t)wp) >> 32);
+    const uint32_t q = (uint32_t)(t >> 32);
+    return (uint64_t)a * w - (uint64_t)q * Prime<PI>::P;
+}
+
 /* Montgomery product a*b*2^-64 mod p for a*b < 2^64 * p; result in [0, p). */
 template <int PI>
 __host__ __device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b)

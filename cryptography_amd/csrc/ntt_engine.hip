@@ -298,7 +298,8 @@ void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t
 /* Enqueue one size class: nprod products, transform length 2^logL, all on
  * `st`.  d_desc == nullptr means the single product `one`. */
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
-                      uint32_t max_nout, int square, hipStream_t st, int residues_in, uint32_t b_period)
+                      uint32_t max_nout, int square, hipStream_t st, int residues_in, uint32_t b_period,
+                      uint32_t max_nab)
 {
     const size_t L = (size_t)1 << logL;
     const size_t len = row_len(logL, nprod);
@@ -354,6 +355,9 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     for (int i = 0; i < pp.npass - 1; i++) {
         geom(i, done);
         pa.first = i == 0 && !residues_in;
+        /* operands of at most L/2 coefficients leave the first pass's
+         * columns zero from row S/2 on */
+        pa.half_zero = pa.first && max_nab && (uint64_t)8 * max_nab <= L;
         int kind = pa.first ? 3 : 0;
         pa.xtab = nullptr;
         if (kind == 0 && (pa.xtab = xtab_for((int)pp.logS[i], (int)pa.logT, st))) kind = 6;
@@ -378,6 +382,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     const bool scale_late = pp.npass >= 2 && pp.logS[0] >= 3 && pa.logC + pp.logS[0] == 12;
     geom(pp.npass - 1, done);
     pa.xtab = nullptr;
+    pa.half_zero = 0;
     pa.first = pp.npass == 1 && !residues_in;
     pa.b_tiles = share ? (uint32_t)(b_period * (L / FT_TILE)) : 0;
     pa.scale_late = scale_late;
@@ -490,7 +495,8 @@ int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d
     class_needs(logL, 1, (uint32_t)nout, square, rw, sb);
     int rc = ws_reserve(g_ws, rw, sb);
     if (rc) return rc;
-    rc = enqueue_ntt_class(g_ws, nullptr, one, 1, logL, (uint32_t)nout, square, g.stream);
+    rc = enqueue_ntt_class(g_ws, nullptr, one, 1, logL, (uint32_t)nout, square, g.stream, 0, 0,
+                           (uint32_t)std::max(na, nb));
     g.timing_pending = g.timing;
     return rc;
 }
@@ -510,11 +516,11 @@ int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
      * batch order either way */
     const unsigned nch = std::max<unsigned>(1, std::min<unsigned>(host_threads(), (unsigned)(n >> 15)));
     std::vector<size_t> ccnt((size_t)nch * NK, 0);
-    std::vector<uint32_t> cmax((size_t)nch * NK, 0);
+    std::vector<uint32_t> cmax((size_t)nch * NK, 0), cmab((size_t)nch * NK, 0);
     std::vector<size_t> cbad(nch, SIZE_MAX);
     auto classify = [&](size_t lo, size_t hi, unsigned w) {
         size_t *cnt = &ccnt[(size_t)w * NK];
-        uint32_t *mx = &cmax[(size_t)w * NK];
+        uint32_t *mx = &cmax[(size_t)w * NK], *mab = &cmab[(size_t)w * NK];
         for (size_t i = lo; i < hi; i++) {
             MulDesc d;
             memcpy(&d, &h[i], sizeof d);
@@ -527,6 +533,7 @@ int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
             key[i] = (uint8_t)(k + 1);
             cnt[k + 1]++;
             mx[k + 1] = std::max(mx[k + 1], std::max(d.n_out, d.na + d.nb));
+            mab[k + 1] = std::max(mab[k + 1], std::max(d.na, d.nb));
         }
     };
     if (nch > 1) {
@@ -545,11 +552,12 @@ int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
     /* first slot of key k for chunk w: all chunks' keys below k, then the
      * earlier chunks' key k */
     size_t total[NK] = {};
-    uint32_t maxo[NK] = {};
+    uint32_t maxo[NK] = {}, maxab[NK] = {};
     for (unsigned w = 0; w < nch; w++)
         for (int k = 0; k < NK; k++) {
             total[k] += ccnt[(size_t)w * NK + k];
             maxo[k] = std::max(maxo[k], cmax[(size_t)w * NK + k]);
+            maxab[k] = std::max(maxab[k], cmab[(size_t)w * NK + k]);
         }
     size_t first[NK + 1] = {};
     for (int k = 0; k < NK; k++) first[k + 1] = first[k] + total[k];
@@ -564,7 +572,10 @@ int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
     bp.sorted.resize(n);
     bp.classes.clear();
     for (int k = 0; k < NK; k++)
-        if (total[k]) bp.classes.push_back({k - 1, (uint32_t)first[k], (uint32_t)total[k], maxo[k]});
+        if (total[k]) {
+            bp.classes.push_back({k - 1, (uint32_t)first[k], (uint32_t)total[k], maxo[k]});
+            bp.classes.back().max_nab = maxab[k];
+        }
     auto scatter = [&](size_t lo, size_t hi, unsigned w) {
         size_t *sl = &slot[(size_t)w * NK];
         for (size_t i = lo; i < hi; i++) memcpy(&bp.sorted[sl[key[i]]++], &h[i], sizeof(MulDesc));
@@ -621,7 +632,7 @@ int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hip
         } else if (c.logL == 0) {
             rc = enqueue_schoolbook(dd, MulDesc{}, c.count, st, ws.flag);
         } else {
-            rc = enqueue_ntt_class(ws, dd, MulDesc{}, c.count, c.logL, c.max_nout, 0, st, 0, c.b_period);
+            rc = enqueue_ntt_class(ws, dd, MulDesc{}, c.count, c.logL, c.max_nout, 0, st, 0, c.b_period, c.max_nab);
         }
         if (rc) return rc;
     }

@@ -391,6 +391,48 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
     }
 }
 
+/* The first DIF round of a first pass whose rows S/2.. are zero (operands
+ * of at most L/2 coefficients, LOGS >= 4): the top stage pairs each 16-bit
+ * input with a zero, so it keeps x and forms x*w (a small-operand Shoup
+ * product, no sum or difference); the round's other two stages as usual.
+ * The zero rows are not read. */
+template <int PI, int LOGS>
+__device__ __forceinline__ void dif_round_hz(uint64_t *lds, const TwPair *twf, uint32_t logC)
+{
+    constexpr int LH = LOGS - 1, RL = 3, R = 8, LD = LH - (RL - 1);
+    constexpr uint32_t d = 1u << LD;
+    static_assert(LOGS >= 4, "the first round is not the last");
+    for (uint32_t gid = threadIdx.x; gid < (FT_SIZE >> RL); gid += FTPB) {
+        const uint32_t c = gid & ((1u << logC) - 1);
+        const uint32_t u = gid >> logC;
+        const uint32_t j = u & (d - 1);
+        const uint32_t gg = u >> LD;                    /* one block per group: the top stage spans S */
+        const uint32_t base = (((gg << LOGS) + j) << logC) + c;
+        uint64_t x[R];
+#pragma unroll
+        for (int i = 0; i < R / 2; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
+#pragma unroll
+        for (int i = 0; i < R / 2; i++) {
+            const TwPair w = twf[(1u << LH) - 1 + j + (uint32_t)(i << LD)];
+            x[i + R / 2] = shoup_small<PI>((uint32_t)x[i], w.w, w.wp);
+        }
+#pragma unroll
+        for (int s = 1; s < RL; s++) {
+            const int half = R >> (s + 1);
+            const int lhs = LH - s;
+#pragma unroll
+            for (int i = 0; i < R; i++) {
+                if (i & half) continue;
+                const uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
+                const TwPair w = twf[(1u << lhs) - 1 + e];
+                dif_bfly<PI>(x[i], x[i + half], w.w, w.wp);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; i++) lds[LP(base + ((uint32_t)(i << LD) << logC))] = x[i];
+    }
+}
+
 /* All DIF rounds of an S = 2^LOGS transform, top stage first, radix 8 while
  * three or more stages remain. */
 template <int PI, int LOGS, int LH, bool XT = false>
@@ -589,7 +631,17 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
             fill_first_tile<LOGS>(pa, op, g, tt, lds);
             lds_sync();
         }
-        dif_all<PI, LOGS, LOGS - 1, !XTAB>(lds, twf, g.logC, xc);   /* chain: cross twiddles in the last round */
+        if constexpr (FIRST && LOGS >= 4) {
+            /* only the first round differs: the rounds after it are shared
+             * code (a second copy of them costs more in instruction fetch
+             * than the cheaper round saves) */
+            if (pa.half_zero) dif_round_hz<PI, LOGS>(lds, twf, g.logC);   /* a bare twiddle product on top */
+            else dif_round<PI, LOGS, LOGS - 1, 3, false>(lds, twf, g.logC, xc);
+            lds_sync();
+            dif_all<PI, LOGS, LOGS - 4, true>(lds, twf, g.logC, xc);
+        } else {
+            dif_all<PI, LOGS, LOGS - 1, !XTAB>(lds, twf, g.logC, xc);   /* chain: cross twiddles in the last round */
+        }
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         {

@@ -60,6 +60,8 @@ struct PassArgs {
     uint64_t scale[3], scale_p[3];   /* 2^64 / L mod p and Shoup partners (middle pass) */
     uint64_t scale_m[3];        /* the same times 2^64 mod p (Montgomery form) */
     int scale_late;             /* middle pass: leave the 1/L scale to the last inverse pass */
+    int half_zero;              /* first pass: every operand has at most L/2 coefficients, so
+                                   rows S/2.. of each column are zero */
     int iscale;                 /* kind 5: multiply by 1/L (the chain's first twiddle carries it);
                                    kind 7 reads a table that includes it */
     const uint64_t *xtab;       /* kinds 6 and 7: this pass's cross twiddles, one

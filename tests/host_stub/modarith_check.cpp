@@ -2,7 +2,7 @@
  * Host check of the modular primitives in cryptography_amd/csrc/modarith.h
  * against 128-bit integer arithmetic, for the three NTT primes: the value and
  * the documented output range of the Montgomery product, the Shoup product
- * and both butterflies, and the compile-time roots.  Prints "bad N" and
+ * (and its small-operand form), both butterflies, and the compile-time roots.  Prints "bad N" and
  * exits non-zero on any mismatch.
  */
 #include "../../cryptography_amd/csrc/modarith.h"
@@ -54,6 +54,13 @@ static void check(long iters)
         const uint64_t r = shoup_lazy<PI>(a, w, wp);
         expect(r < 2 * p && r % p == (uint64_t)((u128)a * w % p), "shoup", PI, a, w);
     };
+    /* small-operand Shoup: the same quotient as the full one, so the same value */
+    auto shoup_s = [&](uint32_t a, uint64_t w) {
+        const uint64_t wp = (uint64_t)(((u128)w << 64) / p);
+        const uint64_t r = shoup_small<PI>(a, w, wp);
+        expect(r < 2 * p && r % p == (uint64_t)((u128)a * w % p), "shoup small", PI, a, w);
+        expect(r == shoup_lazy<PI>(a, w, wp), "shoup small quotient", PI, a, w);
+    };
     auto bfly = [&](uint64_t x, uint64_t y, uint64_t w) {
         const uint64_t wp = (uint64_t)(((u128)w << 64) / p);
         uint64_t a = x, b = y;
@@ -72,12 +79,15 @@ static void check(long iters)
         for (uint64_t b : edges) {
             if (a < 4 * p && b < 2 * p) mont(a, b);
             if (b < p) shoup(a, b);
+            if (b < p && a <= 0xffffffffull) shoup_s((uint32_t)a, b);
             if (a < 2 * p && b < 2 * p) bfly(a, b, (b % p) ? b % p : 1);
         }
     for (long i = 0; i < iters; i++) {
         mont(rnd() % (4 * p), rnd() % (2 * p));
         mont(rnd() % (2 * p), rnd() % (2 * p));
         shoup(rnd(), rnd() % p);
+        shoup_s((uint32_t)rnd(), rnd() % p);
+        shoup_s((uint32_t)(rnd() & 0xffff), rnd() % p);
         bfly(rnd() % (2 * p), rnd() % (2 * p), rnd() % p);
     }
     /* compile-time roots: w of order exactly 2^LOGORD, and w^E */
