@@ -522,15 +522,23 @@ __device__ __forceinline__ void fill_first_tile(const PassArgs &pa, int op, cons
     uint32_t tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     if (g.logC >= 2) {
-        uint64_t limb[FT_PER / 4];
+        /* with one group per tile (C * S = 4096), limb share k covers tile
+         * elements [2048k, 2048(k+1)), rows [k S/2, (k+1) S/2): the second
+         * share is rows S/2.., which the half-zero first round (LOGS >= 4)
+         * never reads */
+        constexpr int NK = FT_PER / 4;
+        static_assert(NK == 2, "two limb shares per thread");
+        const int nk = (LOGS >= 4 && pa.half_zero && g.logG == 0) ? NK / 2 : NK;
+        uint64_t limb[NK];
 #pragma unroll
-        for (int k = 0; k < FT_PER / 4; k++) {
+        for (int k = 0; k < NK; k++) {
             uint32_t q = (tid + k * FTPB) << 2, m, r;
             size_t a = ft_addr(LOGS, g, t, q, m, r);
-            limb[k] = first_limb(pa, op, a);
+            limb[k] = k < nk ? first_limb(pa, op, a) : 0;
         }
 #pragma unroll
-        for (int k = 0; k < FT_PER / 4; k++) {
+        for (int k = 0; k < NK; k++) {
+            if (k >= nk) break;
             uint32_t q = (tid + k * FTPB) << 2;
 #pragma unroll
             for (int i = 0; i < 4; i++) lds[LP(q + i)] = (limb[k] >> (16 * i)) & 0xffff;

@@ -243,10 +243,22 @@ __global__ void __launch_bounds__(256) k_garner_pack(GarnerArgs ga)
 }
 
 /* K_b (grid: products): exclusive carry-in of every block of one product. */
+/* Cells of up to CS_LDS blocks are staged in LDS: read and written back with
+ * coalesced, unrolled byte accesses, so each thread's serial walk over its
+ * segment reads LDS instead of issuing dependent global loads. */
+#define CS_LDS 32768
 __global__ void __launch_bounds__(1024) k_carry_scan(uint8_t *blkall, uint32_t nblk, uint8_t cin0)
 {
     __shared__ uint8_t sg[1024];
-    uint8_t *blk = blkall + (size_t)blockIdx.x * nblk;
+    __shared__ uint8_t cells[CS_LDS];
+    uint8_t *gblk = blkall + (size_t)blockIdx.x * nblk;
+    const bool staged = nblk <= CS_LDS;
+    if (staged) {
+#pragma unroll 8
+        for (uint32_t i = threadIdx.x; i < nblk; i += 1024) cells[i] = gblk[i];
+        __syncthreads();
+    }
+    uint8_t *blk = staged ? cells : gblk;
     const uint32_t per = (nblk + 1023) / 1024;
     const uint32_t s0 = threadIdx.x * per;
     uint8_t agg = 2;
@@ -261,6 +273,11 @@ __global__ void __launch_bounds__(1024) k_carry_scan(uint8_t *blkall, uint32_t n
         uint8_t c = blk[i];
         blk[i] = cin;                 /* becomes the block's carry-in */
         cin = (c & 1) | (((c >> 1) & 1) & cin);
+    }
+    if (staged) {
+        __syncthreads();
+#pragma unroll 8
+        for (uint32_t i = threadIdx.x; i < nblk; i += 1024) gblk[i] = cells[i];
     }
 }
 
