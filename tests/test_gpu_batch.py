@@ -155,3 +155,40 @@ def test_batch_shared_operand(nb, na_list):
             assert np.array_equal(pool.download(oc_, len(a_) + len(bb)), O.gmp_mul(a_, bb))
     finally:
         pool.close()
+
+
+@pytest.mark.parametrize("count,na,nb", [(16, 3000, 2000), (8, 1 << 18, (1 << 18) - 77)])
+def test_batch_table_passes_many_products(count, na, nb):
+    """A size class of `count` products (a multiple of 8 groups): the passes
+    that read cross-twiddle tables take their tiles column slice by column
+    slice across 8 groups, so tiles of different products interleave.
+    L = 2^15 (two passes: the inverse of the first from its table) and
+    L = 2^21 (three passes: the inner pass and its inverse).  Eager and graph
+    runs, tables on and off, all equal to GMP."""
+    pool = EN.DevicePool(count * (2 * (na + nb)) + 4096)
+    old = EN.set_xtab(True)
+    try:
+        descs, host = build_batch(pool, [(na, nb, 0)] * count, seed=61 + count)
+        want = [O.gmp_mul(a, b) for (a, b, _, _, _) in host]
+        for on in (True, False):
+            EN.set_xtab(on)
+            for (_, _, _, oc, n_out) in host:
+                pool.upload(oc, np.zeros(n_out, np.uint64))
+            EN.mul_batch_device(descs)
+            for (_, _, _, oc, n_out), w in zip(host, want):
+                assert np.array_equal(pool.download(oc, n_out), w), f"eager, tables {on}"
+        EN.set_xtab(True)
+        g = EN.BatchGraph(descs)
+        try:
+            for rep in range(2):
+                for (_, _, _, oc, n_out) in host:
+                    pool.upload(oc, np.full(n_out, 0xDEADBEEF, np.uint64))
+                g.launch()
+                EN.sync()
+                for (_, _, _, oc, n_out), w in zip(host, want):
+                    assert np.array_equal(pool.download(oc, n_out), w), f"graph replay {rep}"
+        finally:
+            g.close()
+    finally:
+        EN.set_xtab(old)
+        pool.close()
