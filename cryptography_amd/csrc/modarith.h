@@ -192,6 +192,18 @@ __host__ __device__ __forceinline__ void dit_bfly(uint64_t &x, uint64_t &y, uint
     y = u - t + Prime<PI>::P2;
 }
 
+/* The same butterfly for twiddle w^-e given W = w^(hs-e) = -w^-e (w of order
+ * 2hs): y*w^-e = -y*W, so the sum and difference trade places.  Ranges as
+ * dit_bfly. */
+template <int PI>
+__host__ __device__ __forceinline__ void dit_bfly_neg(uint64_t &x, uint64_t &y, uint64_t W, uint64_t Wp)
+{
+    const uint64_t t = shoup_lazy<PI>(y, W, Wp);
+    const uint64_t u = reduce2<PI>(x);
+    x = u - t + Prime<PI>::P2;
+    y = u + t;
+}
+
 /* The same with twiddle 1: both inputs reduced to [0, 2p). */
 template <int PI>
 __host__ __device__ __forceinline__ void dit_bfly_one(uint64_t &x, uint64_t &y)

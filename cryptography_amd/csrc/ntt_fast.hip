@@ -28,7 +28,9 @@
  *     load-chain inverse twiddles in registers: both load through VGPRs);
  *     block barriers wait for LDS work only, so stores stay in flight.
  * A tile is 4096 residues (32 KB of LDS) worked on by 512 threads: with 8 KB
- * of stage twiddles, three blocks share a CU (registers bound the fourth).
+ * of stage twiddles, three blocks share a CU (registers bound the fourth);
+ * the fused middle pass keeps one twiddle table for both directions and, at
+ * 64 VGPRs, four.
  * Rows of a size class are padded to a multiple of 4096 residues, and residue
  * index a lives at res_at(a, pad_log) (a pad after every first-pass row).
  *
@@ -290,12 +292,28 @@ __device__ __forceinline__ void xt_lookup(XtwCtx &xc, uint32_t logC)
     xc.tb = tw_mont<PI>(xc.tw, eb & ((1u << NTT_ROOT_LOG) - 1));
 }
 
+/* ---- stage twiddles of the fused passes: one table, both directions --------
+ * The fused passes (kinds 2 and 4) run forward and inverse rounds over one
+ * table.  Stage l (half-size hs = 2^l) of the rounds that read LDS (l >= RLL;
+ * the smallest RLL stages use compile-time twiddles) keeps w_(2hs)^k for
+ * k = 0 .. hs at [mid_off(l), mid_off(l) + hs]: the forward twiddle w^e at
+ * +e, and the inverse twiddle w^-e = -w^(hs-e) as the entry at +(hs - e)
+ * with the butterfly's outputs swapped (dit_bfly_neg; +hs holds w^hs = -1,
+ * for e = 0).  510 entries for S = 512: with the 32 KB tile a block needs
+ * 40 KB of LDS, and four blocks share a CU. */
+template <int RLL>
+__host__ __device__ constexpr uint32_t mid_off(int l)
+{
+    return ((1u << l) - (1u << RLL)) + (uint32_t)(l - RLL);
+}
+
 /* ---- radix-R rounds over the LDS tile -------------------------------------
  * A DIF round with top stage half-size h = 2^LH and R = 2^RL covers stages
  * h, h/2, .., h/2^(RL-1): the groups {m0 + i*d}, d = h >> (RL-1),
  * m0 = blk*2h + j, j < d. */
 
-template <int PI, int LOGS, int LH, int RL, bool XT>
+/* MIDRLL > 0: the fused passes' shared table (mid_off<MIDRLL>). */
+template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0>
 __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint32_t logC, const XtwCtx &xc)
 {
     constexpr int R = 1 << RL;
@@ -333,7 +351,7 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
                 for (int i = 0; i < R; i++) {
                     if (i & half) continue;
                     uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);   /* (m mod hs) */
-                    TwPair w = twf[(1u << lhs) - 1 + e];
+                    TwPair w = MIDRLL ? twf[mid_off<MIDRLL ? MIDRLL : 1>(lhs) + e] : twf[(1u << lhs) - 1 + e];
                     dif_bfly<PI>(x[i], x[i + half], w.w, w.wp);
                 }
             }
@@ -343,7 +361,7 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
     }
 }
 
-template <int PI, int LOGS, int LH, int RL, bool XT>
+template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0>
 __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint32_t logC, const XtwCtx &xc)
 {
     constexpr int R = 1 << RL;
@@ -381,8 +399,13 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
                 for (int i = 0; i < R; i++) {
                     if (i & half) continue;
                     uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
-                    TwPair w = twi[(1u << lhs) - 1 + e];
-                    dit_bfly<PI>(x[i], x[i + half], w.w, w.wp);
+                    if constexpr (MIDRLL) {     /* -w^(hs-e): the forward entry, outputs swapped */
+                        const TwPair w = twi[mid_off<MIDRLL ? MIDRLL : 1>(lhs) + (1u << lhs) - e];
+                        dit_bfly_neg<PI>(x[i], x[i + half], w.w, w.wp);
+                    } else {
+                        const TwPair w = twi[(1u << lhs) - 1 + e];
+                        dit_bfly<PI>(x[i], x[i + half], w.w, w.wp);
+                    }
                 }
             }
         }
@@ -761,24 +784,45 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
 
 /* DIF rounds of an S = 2^LOGS transform except the last (smallest) one, and
  * DIT rounds except the first: the fused middle pass does that round itself. */
-template <int PI, int LOGS, int LH>
-__device__ __forceinline__ void dif_upper(uint64_t *lds, const TwPair *twf)
+template <int PI, int LOGS, int LH, int RLL>
+__device__ __forceinline__ void dif_upper(uint64_t *lds, const TwPair *tw)
 {
     constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
     if constexpr (LH - RL >= 0) {
-        dif_round<PI, LOGS, LH, RL, false>(lds, twf, 0, XtwCtx{});
+        dif_round<PI, LOGS, LH, RL, false, RLL>(lds, tw, 0, XtwCtx{});
         lds_sync();
-        dif_upper<PI, LOGS, LH - RL>(lds, twf);
+        dif_upper<PI, LOGS, LH - RL, RLL>(lds, tw);
     }
 }
-template <int PI, int LOGS, int LH>
-__device__ __forceinline__ void dit_upper(uint64_t *lds, const TwPair *twi)
+template <int PI, int LOGS, int LH, int RLL>
+__device__ __forceinline__ void dit_upper(uint64_t *lds, const TwPair *tw)
 {
     constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
     if constexpr (LH - RL >= 0) {
-        dit_upper<PI, LOGS, LH - RL>(lds, twi);
-        dit_round<PI, LOGS, LH, RL, false>(lds, twi, 0, XtwCtx{});
+        dit_upper<PI, LOGS, LH - RL, RLL>(lds, tw);
+        dit_round<PI, LOGS, LH, RL, false, RLL>(lds, tw, 0, XtwCtx{});
         lds_sync();
+    }
+}
+
+/* The fused passes' table (layout above): entry k of stage l is
+ * w_(2hs)^k = w_4096^(k * 4096 / (2hs)) for k < hs, and -1 for k = hs. */
+template <int PI, int LOGS, int RLL>
+__device__ __forceinline__ void stage_twiddles_mid(const TwPair *small_all, TwPair *tw)
+{
+    const TwPair *sm = small_all + PI * (FT_SIZE / 2);
+    constexpr uint32_t n = mid_off<RLL>(LOGS);
+    for (uint32_t idx = threadIdx.x; idx < n; idx += FTPB) {
+        int l = RLL;
+        while (l + 1 < LOGS && idx >= mid_off<RLL>(l + 1)) l++;
+        const uint32_t k = idx - mid_off<RLL>(l);
+        if (k < (1u << l)) {
+            tw[idx] = sm[k << (FT_LOG - 1 - l)];
+        } else {       /* w^hs = -1: p - 1, Shoup partner 2^64 - 1 - partner(1) */
+            const TwPair one = sm[0];
+            tw[idx].w = Prime<PI>::P - one.w;
+            tw[idx].wp = ~one.wp;
+        }
     }
 }
 
@@ -789,8 +833,7 @@ __device__ __forceinline__ void dit_upper(uint64_t *lds, const TwPair *twi)
  * round's layout, and b's last DIF round, the product and the first DIT
  * round run on the same registers without an LDS round trip. */
 template <int PI, int LOGS, bool FIRST>
-__device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, uint64_t *lds, TwPair *twf,
-                         TwPair *twi)
+__device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, uint64_t *lds, TwPair *tw)
 {
     constexpr int RLL = ((LOGS - 1) % 3) + 1;       /* the smallest round: 2^RLL points per group */
     constexpr int RR = 1 << RLL;
@@ -802,7 +845,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     uint64_t ra[FT_PER];
     size_t base = 0, step = 0;
     if constexpr (FIRST) {
-        stage_twiddles<PI, LOGS, 3>(pa.small_tw, twf, twi);
+        stage_twiddles_mid<PI, LOGS, RLL>(pa.small_tw, tw);
 #pragma unroll
         for (int k = 0; k < FT_PER; k++) {
             uint32_t q = threadIdx.x + k * FTPB, m, r;
@@ -811,11 +854,11 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     } else {
         /* a's tile is in flight while the stage twiddles are written */
         tile_dma<LOGS>(pa, xa, g, t, lds);
-        stage_twiddles<PI, LOGS, 3>(pa.small_tw, twf, twi);
+        stage_twiddles_mid<PI, LOGS, RLL>(pa.small_tw, tw);
         WAIT_VMCNT(0);
     }
     lds_sync();
-    dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
+    dif_upper<PI, LOGS, LOGS - 1, RLL>(lds, tw);
     /* a's smallest round, kept in registers: its groups leave LDS first */
 #pragma unroll
     for (int n = 0; n < NG; n++) {
@@ -852,7 +895,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     if (!pa.square) {
         if constexpr (!FIRST) WAIT_VMCNT(0);
         lds_sync();
-        dif_upper<PI, LOGS, LOGS - 1>(lds, twf);
+        dif_upper<PI, LOGS, LOGS - 1, RLL>(lds, tw);
     }
     const uint64_t s = pa.scale[PI], sp = pa.scale_p[PI];
 #pragma unroll
@@ -880,7 +923,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
         for (int i = 0; i < RR; i++) lds[LP(base + i)] = x[i];
     }
     lds_sync();
-    dit_upper<PI, LOGS, LOGS - 1>(lds, twi);
+    dit_upper<PI, LOGS, LOGS - 1, RLL>(lds, tw);
     tile_span<LOGS>(pa, g, t, threadIdx.x, base, step);
 #pragma unroll
     for (int k = 0; k < FT_PER; k++) xa[base + k * step] = lds[LP(threadIdx.x + k * FTPB)];
@@ -894,36 +937,49 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
  * 6: kind 0 and 7: kind 5 with the cross twiddles from pa.xtab. */
 template <int PI, int LOGS, int KIND>
 __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, int op,
-                                          uint64_t *lds, TwPair *twf, TwPair *twi)
+                                          uint64_t *lds, TwPair *tw)
 {
     /* each routine stages its stage twiddles itself, after starting its
      * first tile's transfer */
-    if constexpr (KIND == 0) fwd_tiles<PI, LOGS, false, false>(pa, xa, op, lds, twf);
-    else if constexpr (KIND == 6) fwd_tiles<PI, LOGS, false, true>(pa, xa, op, lds, twf);
-    else if constexpr (KIND == 3) fwd_tiles<PI, LOGS, true, false>(pa, xa, op, lds, twf);
-    else if constexpr (KIND == 1) inv_tiles<PI, LOGS, false, false>(pa, xa, lds, twi);
-    else if constexpr (KIND == 5) inv_tiles<PI, LOGS, true, false>(pa, xa, lds, twi);
-    else if constexpr (KIND == 7) inv_tiles<PI, LOGS, true, true>(pa, xa, lds, twi);
-    else if constexpr (KIND == 2) mid_tile<PI, LOGS, false>(pa, xa, xb, lds, twf, twi);
-    else mid_tile<PI, LOGS, true>(pa, xa, xb, lds, twf, twi);
+    if constexpr (KIND == 0) fwd_tiles<PI, LOGS, false, false>(pa, xa, op, lds, tw);
+    else if constexpr (KIND == 6) fwd_tiles<PI, LOGS, false, true>(pa, xa, op, lds, tw);
+    else if constexpr (KIND == 3) fwd_tiles<PI, LOGS, true, false>(pa, xa, op, lds, tw);
+    else if constexpr (KIND == 1) inv_tiles<PI, LOGS, false, false>(pa, xa, lds, tw);
+    else if constexpr (KIND == 5) inv_tiles<PI, LOGS, true, false>(pa, xa, lds, tw);
+    else if constexpr (KIND == 7) inv_tiles<PI, LOGS, true, true>(pa, xa, lds, tw);
+    else if constexpr (KIND == 2) mid_tile<PI, LOGS, false>(pa, xa, xb, lds, tw);
+    else mid_tile<PI, LOGS, true>(pa, xa, xb, lds, tw);
 }
 
+/* Stage-twiddle entries in LDS: one direction's S - 1 (forward or inverse
+ * passes), or the fused passes' shared table. */
 template <int LOGS, int KIND>
-__global__ void __launch_bounds__(FTPB) k_fast_pass(PassArgs pa)
+constexpr uint32_t tw_lds_entries()
 {
-    /* stage twiddles: forward only, inverse only, or both (fused passes) */
-    constexpr int NTW = (KIND == 2 || KIND == 4) ? 2 : 1;
+    /* (at least one entry: transforms of up to 8 points have no LDS stages) */
+    return (KIND == 2 || KIND == 4) ? (mid_off<((LOGS - 1) % 3) + 1>(LOGS) ? mid_off<((LOGS - 1) % 3) + 1>(LOGS) : 1)
+                                    : (1u << LOGS);
+}
+
+/* The fused middle pass of 512 points (kind 2, S = 2^9: the passes of the
+ * longest products) asks for 8 waves per SIMD: at 40 KB of LDS four blocks
+ * fit a CU, and the register allocation keeps it at 64 VGPRs without
+ * spilling.  Everything else keeps its allocation (capped at 64 VGPRs the
+ * other kinds, and the fused pass at other sizes, spill). */
+template <int LOGS, int KIND>
+__global__ void __launch_bounds__(FTPB) __attribute__((amdgpu_waves_per_eu(KIND == 2 && LOGS == 9 ? 8 : 1)))
+k_fast_pass(PassArgs pa)
+{
     __shared__ uint64_t lds[FT_LDS];
-    __shared__ TwPair tws[NTW << LOGS];
+    __shared__ TwPair tws[tw_lds_entries<LOGS, KIND>()];
 
     static_assert((1 << LOGS) <= TW_LDS_MAX, "stage twiddles exceed LDS table");
-    TwPair *twf = tws, *twi = tws + (NTW - 1) * (1 << LOGS);
     const uint32_t pi = blockIdx.y, op = blockIdx.z + pa.op0;
     uint64_t *xa = pa.res + ((size_t)op * NTT_NPRIMES + pi) * pa.len;
     const uint64_t *xb = pa.res + ((size_t)NTT_NPRIMES + pi) * pa.len;
-    if (pi == 0) pass_body<0, LOGS, KIND>(pa, xa, xb, op, lds, twf, twi);
-    else if (pi == 1) pass_body<1, LOGS, KIND>(pa, xa, xb, op, lds, twf, twi);
-    else pass_body<2, LOGS, KIND>(pa, xa, xb, op, lds, twf, twi);
+    if (pi == 0) pass_body<0, LOGS, KIND>(pa, xa, xb, op, lds, tws);
+    else if (pi == 1) pass_body<1, LOGS, KIND>(pa, xa, xb, op, lds, tws);
+    else pass_body<2, LOGS, KIND>(pa, xa, xb, op, lds, tws);
 }
 
 /* Host-side launcher: one pass of a class (grid.x tiles, grid.y primes,

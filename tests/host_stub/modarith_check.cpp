@@ -2,7 +2,8 @@
  * Host check of the modular primitives in cryptography_amd/csrc/modarith.h
  * against 128-bit integer arithmetic, for the three NTT primes: the value and
  * the documented output range of the Montgomery product, the Shoup product
- * (and its small-operand form), both butterflies, and the compile-time roots.  Prints "bad N" and
+ * (and its small-operand form), both butterflies (the inverse one also with a
+ * negated twiddle), and the compile-time roots.  Prints "bad N" and
  * exits non-zero on any mismatch.
  */
 #include "../../cryptography_amd/csrc/modarith.h"
@@ -74,6 +75,13 @@ static void check(long iters)
         const uint64_t t = (uint64_t)((u128)(Y % p) * w % p);
         expect(a < 4 * p && a % p == (X % p + t) % p, "dit sum", PI, X, Y);
         expect(b < 4 * p && b % p == (X % p + p - t) % p, "dit diff", PI, X, Y);
+        /* negated-twiddle form: W = -w gives the butterfly of twiddle w */
+        const uint64_t W = (p - w) % p, Wp = (uint64_t)(((u128)W << 64) / p);
+        a = X;
+        b = Y;
+        dit_bfly_neg<PI>(a, b, W, Wp);
+        expect(a < 4 * p && a % p == (X % p + t) % p, "dit neg sum", PI, X, Y);
+        expect(b < 4 * p && b % p == (X % p + p - t) % p, "dit neg diff", PI, X, Y);
     };
     for (uint64_t a : edges)
         for (uint64_t b : edges) {
