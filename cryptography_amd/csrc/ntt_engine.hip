@@ -325,8 +325,16 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     pa.square = square;
     pa.ntiles = (uint32_t)ntiles;
     /* several tiles per block once the grid has blocks to spare (about 4096
-     * blocks per prime row fill the chip several times over) */
-    pa.tpb = (uint32_t)std::min<size_t>(std::max<size_t>(ntiles * NTT_NPRIMES / 4096, 1), (size_t)g.tile_batch);
+     * blocks per prime row fill the chip several times over); the forward
+     * table pass (kind 6) takes at least two: its consecutive tiles share a
+     * column slice, and the second one halves the stage-twiddle staging
+     * (measured 3.75 -> 3.62 ms at L = 2^27; the other kinds are slower with
+     * two) */
+    auto tiles_per_block = [&](size_t nt, int kind) {
+        const size_t cap = kind == 6 ? std::max(g.tile_batch, 2) : g.tile_batch;
+        return (uint32_t)std::min<size_t>(std::max<size_t>(nt * NTT_NPRIMES / 4096, 1), cap);
+    };
+    pa.tpb = tiles_per_block(ntiles, 0);
     for (int i = 0; i < NTT_NPRIMES; i++) {
         uint64_t p = prime(i);
         uint64_t r = (uint64_t)(((u128)1 << 64) % p);
@@ -361,6 +369,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         int kind = pa.first ? 3 : 0;
         pa.xtab = nullptr;
         if (kind == 0 && (pa.xtab = xtab_for((int)pp.logS[i], (int)pa.logT, st))) kind = 6;
+        pa.tpb = tiles_per_block(ntiles, kind);
         HIP_TRY(launch_fast_pass(pp.logS[i], kind, pa, ntiles, fwd_ops, st));
         HIP_TRY(mark("k_fast_pass<fwd>"));
         if (share) {   /* b of the first b_period products */
@@ -368,8 +377,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
             pb.op0 = 1;
             pb.nprod = b_period;
             pb.ntiles = (uint32_t)(b_period * (L / FT_TILE));
-            pb.tpb = (uint32_t)std::min<size_t>(std::max<size_t>((size_t)pb.ntiles * NTT_NPRIMES / 4096, 1),
-                                                (size_t)g.tile_batch);
+            pb.tpb = tiles_per_block(pb.ntiles, kind);
             HIP_TRY(launch_fast_pass(pp.logS[i], kind, pb, pb.ntiles, 1, st));
             HIP_TRY(mark("k_fast_pass<fwd b>"));
         }
@@ -383,6 +391,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     geom(pp.npass - 1, done);
     pa.xtab = nullptr;
     pa.half_zero = 0;
+    pa.tpb = tiles_per_block(ntiles, 0);
     pa.first = pp.npass == 1 && !residues_in;
     pa.b_tiles = share ? (uint32_t)(b_period * (L / FT_TILE)) : 0;
     pa.scale_late = scale_late;
