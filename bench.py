@@ -105,11 +105,20 @@ def launch_table(L, logL, na, nb):
     Lw = 1 << logL
     ntiles = max(1, Lw // 4096)
     plan = pass_plan(L, logL)
+    L.ntt_engine_get_tile_batch.restype = ctypes.c_int
+    batch = L.ntt_engine_get_tile_batch()
+
+    def blocks(kind):
+        """Blocks per (operand, prime) row: tiles per block as enqueue_ntt_class sets them."""
+        cap = max(batch, 2) if kind == 6 else batch
+        tpb = min(max(ntiles * K_PRIMES // 4096, 1), cap)
+        return -(-ntiles // tpb)
+
     out, done = [], 0
     for i, s in enumerate(plan[:-1]):
         kind = 3 if i == 0 else (6 if has_xtab(s, logL - done - s) else 0)
         byt = 8 * (na + nb) + 2 * K_PRIMES * 8 * Lw if i == 0 else 2 * K_PRIMES * 16 * Lw
-        out.append((f"k_fast_pass<{s}, {kind}>", ntiles * K_PRIMES * 2, byt))
+        out.append((f"k_fast_pass<{s}, {kind}>", blocks(kind) * K_PRIMES * 2, byt))
         done += s
     out.append((f"k_fast_pass<{plan[-1]}, {2 if len(plan) > 1 else 4}>", ntiles * K_PRIMES, K_PRIMES * 24 * Lw))
     for i in range(len(plan) - 2, -1, -1):
@@ -119,7 +128,7 @@ def launch_table(L, logL, na, nb):
         kind = 5 if plan[i] >= 3 and logC + plan[i] == 12 else 1
         if kind == 5 and has_xtab(plan[i], logT):
             kind = 7
-        out.append((f"k_fast_pass<{plan[i]}, {kind}>", ntiles * K_PRIMES, K_PRIMES * 16 * Lw))
+        out.append((f"k_fast_pass<{plan[i]}, {kind}>", blocks(kind) * K_PRIMES, K_PRIMES * 16 * Lw))
     out.append(("k_garner_pack", None, K_PRIMES * 8 * Lw + 8 * (na + nb)))
     return out
 
