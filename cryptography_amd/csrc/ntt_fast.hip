@@ -17,10 +17,14 @@
  *     passes whose S x T block has at most 2^20 residues (the inner pass of
  *     a three-pass transform, the inverse of a two-pass one) instead read one
  *     precomputed Montgomery-form twiddle per residue (kinds 6 and 7);
- *   - the last forward pass of both operands, the pointwise product (with the
- *     1/L scale) and the first inverse pass are one kernel (kind 2): both
- *     spectra of a tile are formed, multiplied and transformed back without
- *     leaving the chip;
+ *   - the last forward pass of both operands, the pointwise product and the
+ *     first inverse pass are one kernel (kind 2): both spectra of a tile are
+ *     formed, multiplied and transformed back without leaving the chip (the
+ *     1/L scale rides on the last inverse pass's cross twiddles when that
+ *     pass applies them at the load, else on the pointwise product);
+ *   - a first pass whose operands fill at most half the transform (every
+ *     balanced product) skips the zero upper half of its columns: the top
+ *     stage is a bare product of 16-bit inputs by their twiddles;
  *   - the forward column pass, the in-round inverse and the fused pass take
  *     their tiles into LDS by DMA (global_load_lds_dwordx4, no registers),
  *     writing the stage twiddles and looking up the cross-twiddle bases while
