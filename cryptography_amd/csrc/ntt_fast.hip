@@ -149,7 +149,9 @@ __device__ __forceinline__ void tile_span(const PassArgs &pa, const FGeom &g, ui
  * immediates here, and e = 0 (a quarter to all of a stage's butterflies)
  * skips the multiply.  Stage s of the round has half-size hs = 2^(RL-1-s),
  * root order 2^(RL-s), pair distance 2^(RL-1-s) in the group. */
-template <int PI, int RL, int S, int I, bool INV>
+/* LAZY (forward, last stage): the twiddle-1 butterflies leave x + y and
+ * x - y + 2p unreduced, in [0, 4p). */
+template <int PI, int RL, int S, int I, bool INV, bool LAZY = false>
 __device__ __forceinline__ void const_stage(uint64_t *x)
 {
     constexpr int R = 1 << RL;
@@ -159,8 +161,15 @@ __device__ __forceinline__ void const_stage(uint64_t *x)
             constexpr int e = I & (half - 1);
             constexpr int lo = RL - S;
             if constexpr (e == 0) {
-                if constexpr (INV) dit_bfly_one<PI>(x[I], x[I + half]);
-                else bfly_one<PI>(x[I], x[I + half]);
+                if constexpr (INV) {
+                    dit_bfly_one<PI>(x[I], x[I + half]);
+                } else if constexpr (LAZY) {
+                    const uint64_t a = x[I], b = x[I + half];
+                    x[I] = a + b;
+                    x[I + half] = a - b + Prime<PI>::P2;
+                } else {
+                    bfly_one<PI>(x[I], x[I + half]);
+                }
             } else if constexpr (INV) {
                 using T = ConstTw<PI, lo, (1 << lo) - e>;
                 dit_bfly<PI>(x[I], x[I + half], T::w, T::wp);
@@ -169,17 +178,19 @@ __device__ __forceinline__ void const_stage(uint64_t *x)
                 dif_bfly<PI>(x[I], x[I + half], T::w, T::wp);
             }
         }
-        const_stage<PI, RL, S, I + 1, INV>(x);
+        const_stage<PI, RL, S, I + 1, INV, LAZY>(x);
     }
 }
 
-/* DIF: stages top (S = 0) to bottom; DIT: bottom (S = RL-1) to top. */
-template <int PI, int RL, int S>
+/* DIF: stages top (S = 0) to bottom; DIT: bottom (S = RL-1) to top.
+ * LAZYLAST: the last stage's outputs in [0, 4p) (its butterflies all have
+ * twiddle 1). */
+template <int PI, int RL, int S, bool LAZYLAST = false>
 __device__ __forceinline__ void const_dif(uint64_t *x)
 {
     if constexpr (S < RL) {
-        const_stage<PI, RL, S, 0, false>(x);
-        const_dif<PI, RL, S + 1>(x);
+        const_stage<PI, RL, S, 0, false, LAZYLAST && S == RL - 1>(x);
+        const_dif<PI, RL, S + 1, LAZYLAST>(x);
     }
 }
 template <int PI, int RL, int S>
@@ -892,8 +903,10 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
      * pointwise product, after b's rounds) */
 #pragma unroll
     for (int i = 0; i < FT_PER; i++) asm volatile("" : "+v"(ra[i]));
+    /* a's values may stay in [0, 4p): the Montgomery product takes one
+     * factor below 4p when the other is below 2p */
 #pragma unroll
-    for (int n = 0; n < NG; n++) const_dif<PI, RLL, 0>(ra + n * RR);
+    for (int n = 0; n < NG; n++) const_dif<PI, RLL, 0, true>(ra + n * RR);
 #pragma unroll
     for (int i = 0; i < FT_PER; i++) asm volatile("" : "+v"(ra[i]));
     if (!pa.square) {
@@ -907,9 +920,9 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
         const uint32_t gid = threadIdx.x + n * FTPB;
         const uint32_t base = ((gid >> LNB) << LOGS) + ((gid & ((1u << LNB) - 1)) << RLL);
         uint64_t x[RR];
-        if (pa.square) {
+        if (pa.square) {      /* the second factor below 2p */
 #pragma unroll
-            for (int i = 0; i < RR; i++) x[i] = ra[n * RR + i];
+            for (int i = 0; i < RR; i++) x[i] = reduce2<PI>(ra[n * RR + i]);
         } else {
 #pragma unroll
             for (int i = 0; i < RR; i++) x[i] = lds[LP(base + i)];
