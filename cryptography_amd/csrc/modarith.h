@@ -138,7 +138,7 @@ __host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
 }
 
 /* Montgomery product a*b*2^-64 mod p for a*b < 8p^2 (a < 4p and b < 2p);
- * result in (0, 3p), and in (0, 2p) when a, b < 2p.
+ * result in (0, 3p), and in (0, 2p) when a, b < 2p or when a < 4p, b < p.
  * Every prime here is 1 mod 2^32 (p = ph*2^32 + 1), so p^-1 = 1 (mod 2^32)
  * and the reduction is two subtractive 32-bit steps with free multipliers:
  * T = a*b = hi*2^64 + l1*2^32 + l0; T - l0*p = 2^32 (hi*2^32 + D) with
@@ -146,7 +146,7 @@ __host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
  * 2^32 (hi + floor(D / 2^32) - s0*ph).  That last value is
  * (T - m p) / 2^64 for m = l0 + s0*2^32 < 2^64, so it lies in
  * (T/2^64 - p, T/2^64]; adding p gives (T/2^64, T/2^64 + p], and
- * T/2^64 < 8p^2/2^64 < 2p as p < 2^62.  The 128-bit product takes four
+ * T/2^64 < 8p^2/2^64 < 2p as p < 2^62 (below p when T < 4p^2).  The 128-bit product takes four
  * 32x32->64 multiplies and each step one more: no carries to compare. */
 template <int PI>
 __host__ __device__ __forceinline__ uint64_t mont_mul_lazy(uint64_t a, uint64_t b)

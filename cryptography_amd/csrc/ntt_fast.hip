@@ -327,8 +327,9 @@ __host__ __device__ constexpr uint32_t mid_off(int l)
  * h, h/2, .., h/2^(RL-1): the groups {m0 + i*d}, d = h >> (RL-1),
  * m0 = blk*2h + j, j < d. */
 
-/* MIDRLL > 0: the fused passes' shared table (mid_off<MIDRLL>). */
-template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0>
+/* MIDRLL > 0: the fused passes' shared table (mid_off<MIDRLL>).  LAZYLAST:
+ * the smallest round (LD = 0) leaves its last stage's outputs in [0, 4p). */
+template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0, bool LAZYLAST = false>
 __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint32_t logC, const XtwCtx &xc)
 {
     constexpr int R = 1 << RL;
@@ -348,7 +349,7 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
 #pragma unroll
         for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
         if constexpr (LD == 0) {
-            const_dif<PI, RL, 0>(x);
+            const_dif<PI, RL, 0, LAZYLAST>(x);
             if constexpr (XT && xt_ahead<LOGS>()) {
                 chain_apply<PI, RL>(x, xc.ta, xc.tb);
             } else if constexpr (XT) {
@@ -473,14 +474,14 @@ __device__ __forceinline__ void dif_round_hz(uint64_t *lds, const TwPair *twf, u
 
 /* All DIF rounds of an S = 2^LOGS transform, top stage first, radix 8 while
  * three or more stages remain. */
-template <int PI, int LOGS, int LH, bool XT = false>
+template <int PI, int LOGS, int LH, bool XT = false, bool LAZYLAST = false>
 __device__ __forceinline__ void dif_all(uint64_t *lds, const TwPair *twf, uint32_t logC, const XtwCtx &xc = {})
 {
     if constexpr (LH >= 0) {
         constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
-        dif_round<PI, LOGS, LH, RL, XT>(lds, twf, logC, xc);
+        dif_round<PI, LOGS, LH, RL, XT, 0, LAZYLAST>(lds, twf, logC, xc);
         lds_sync();
-        dif_all<PI, LOGS, LH - RL, XT>(lds, twf, logC, xc);
+        dif_all<PI, LOGS, LH - RL, XT, LAZYLAST>(lds, twf, logC, xc);
     }
 }
 
@@ -686,7 +687,10 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
             lds_sync();
             dif_all<PI, LOGS, LOGS - 4, true>(lds, twf, g.logC, xc);
         } else {
-            dif_all<PI, LOGS, LOGS - 1, !XTAB>(lds, twf, g.logC, xc);   /* chain: cross twiddles in the last round */
+            /* chain: cross twiddles in the last round; table: the last stage
+             * stays in [0, 4p), as the store's Montgomery product by a table
+             * entry below p returns (0, 2p) */
+            dif_all<PI, LOGS, LOGS - 1, !XTAB, XTAB>(lds, twf, g.logC, xc);
         }
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));

@@ -49,6 +49,7 @@ static void check(long iters)
         expect((u128)(r % p) * R % p == (u128)a * b % p, "mont value", PI, a, b);
         expect(r > 0 && r < 3 * p, "mont range (a<4p, b<2p)", PI, a, b);
         if (a < 2 * p && b < 2 * p) expect(r < 2 * p, "mont range (a,b<2p)", PI, a, b);
+        if (b < p) expect(r < 2 * p, "mont range (a<4p, b<p)", PI, a, b);
     };
     auto shoup = [&](uint64_t a, uint64_t w) {
         const uint64_t wp = (uint64_t)(((u128)w << 64) / p);
@@ -93,6 +94,7 @@ static void check(long iters)
     for (long i = 0; i < iters; i++) {
         mont(rnd() % (4 * p), rnd() % (2 * p));
         mont(rnd() % (2 * p), rnd() % (2 * p));
+        mont(rnd() % (4 * p), rnd() % p);
         shoup(rnd(), rnd() % p);
         shoup_s((uint32_t)rnd(), rnd() % p);
         shoup_s((uint32_t)(rnd() & 0xffff), rnd() % p);
