@@ -149,8 +149,9 @@ __device__ __forceinline__ void tile_span(const PassArgs &pa, const FGeom &g, ui
  * immediates here, and e = 0 (a quarter to all of a stage's butterflies)
  * skips the multiply.  Stage s of the round has half-size hs = 2^(RL-1-s),
  * root order 2^(RL-s), pair distance 2^(RL-1-s) in the group. */
-/* LAZY (forward, last stage): the twiddle-1 butterflies leave x + y and
- * x - y + 2p unreduced, in [0, 4p). */
+/* LAZY, forward (last stage): the twiddle-1 butterflies leave x + y and
+ * x - y + 2p unreduced, in [0, 4p).  LAZY, inverse (first stage): the inputs
+ * are below 2p already, so the twiddle-1 butterflies skip their reductions. */
 template <int PI, int RL, int S, int I, bool INV, bool LAZY = false>
 __device__ __forceinline__ void const_stage(uint64_t *x)
 {
@@ -161,7 +162,11 @@ __device__ __forceinline__ void const_stage(uint64_t *x)
             constexpr int e = I & (half - 1);
             constexpr int lo = RL - S;
             if constexpr (e == 0) {
-                if constexpr (INV) {
+                if constexpr (INV && LAZY) {
+                    const uint64_t a = x[I], b = x[I + half];
+                    x[I] = a + b;
+                    x[I + half] = a - b + Prime<PI>::P2;
+                } else if constexpr (INV) {
                     dit_bfly_one<PI>(x[I], x[I + half]);
                 } else if constexpr (LAZY) {
                     const uint64_t a = x[I], b = x[I + half];
@@ -193,12 +198,14 @@ __device__ __forceinline__ void const_dif(uint64_t *x)
         const_dif<PI, RL, S + 1, LAZYLAST>(x);
     }
 }
-template <int PI, int RL, int S>
+/* REDUCEDIN: the inputs are below 2p (the first stage, S = RL-1, then skips
+ * its reductions). */
+template <int PI, int RL, int S, bool REDUCEDIN = false>
 __device__ __forceinline__ void const_dit(uint64_t *x)
 {
     if constexpr (S >= 0) {
-        const_stage<PI, RL, S, 0, true>(x);
-        const_dit<PI, RL, S - 1>(x);
+        const_stage<PI, RL, S, 0, true, REDUCEDIN && S == RL - 1>(x);
+        const_dit<PI, RL, S - 1, false>(x);
     }
 }
 
@@ -377,7 +384,8 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
     }
 }
 
-template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0>
+/* REDUCEDIN: the smallest round's (LD = 0) inputs are below 2p. */
+template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0, bool REDUCEDIN = false>
 __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint32_t logC, const XtwCtx &xc)
 {
     constexpr int R = 1 << RL;
@@ -405,7 +413,7 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
                 chain_twiddle<PI, RL>(x, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift),
                                       0u - ((r << (LOGS - RL)) << xc.shift));
             }
-            const_dit<PI, RL, RL - 1>(x);
+            const_dit<PI, RL, RL - 1, REDUCEDIN && !XT>(x);
         } else {
 #pragma unroll
             for (int s = RL - 1; s >= 0; s--) {
@@ -486,13 +494,13 @@ __device__ __forceinline__ void dif_all(uint64_t *lds, const TwPair *twf, uint32
 }
 
 /* DIT rounds in the mirror order (lowest stages first). */
-template <int PI, int LOGS, int LH, bool XT = false>
+template <int PI, int LOGS, int LH, bool XT = false, bool REDUCEDIN = false>
 __device__ __forceinline__ void dit_all(uint64_t *lds, const TwPair *twi, uint32_t logC, const XtwCtx &xc = {})
 {
     if constexpr (LH >= 0) {
         constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
-        dit_all<PI, LOGS, LH - RL, XT>(lds, twi, logC, xc);
-        dit_round<PI, LOGS, LH, RL, XT>(lds, twi, logC, xc);
+        dit_all<PI, LOGS, LH - RL, XT, REDUCEDIN>(lds, twi, logC, xc);
+        dit_round<PI, LOGS, LH, RL, XT, 0, REDUCEDIN>(lds, twi, logC, xc);
         lds_sync();
     }
 }
@@ -778,8 +786,10 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
             for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
             lds_sync();
         }
-        /* otherwise the inverse cross twiddles go into the first round */
-        dit_all<PI, LOGS, LOGS - 1, !LOADCHAIN>(lds, twi, g.logC, xc);
+        /* otherwise the inverse cross twiddles go into the first round; the
+         * table's Montgomery products (an entry below p) leave the tile
+         * below 2p */
+        dit_all<PI, LOGS, LOGS - 1, !LOADCHAIN, XTAB>(lds, twi, g.logC, xc);
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         {
