@@ -294,10 +294,14 @@ def compute_e(n_digits, batched_tree=True):
     init()
     L.ntt_compute_e.argtypes = [ctypes.c_long, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
                                 ctypes.POINTER(EStats)]
-    buf = ctypes.create_string_buffer(n_digits + 2)
+    # an uninitialised buffer and one ASCII decode of it: at 10^8 digits a zeroed ctypes
+    # buffer and its .value copy cost more host time than the division
+    buf = np.empty(n_digits + 2, dtype=np.uint8)
     st = EStats()
-    check(L.ntt_compute_e(n_digits, 1 if batched_tree else 0, buf, n_digits + 2, ctypes.byref(st)))
-    return buf.value.decode(), {k: getattr(st, k) for k, _ in EStats._fields_}
+    check(L.ntt_compute_e(n_digits, 1 if batched_tree else 0, buf.ctypes.data_as(ctypes.c_char_p),
+                          n_digits + 2, ctypes.byref(st)))
+    # the library writes "2." and n_digits - 1 digits, then a NUL
+    return str(memoryview(buf)[:n_digits + 1], "ascii"), {k: getattr(st, k) for k, _ in EStats._fields_}
 
 
 def autotune(min_log=11, max_log=27, save=None, load=None):
