@@ -88,6 +88,20 @@ __host__ __device__ __forceinline__ uint64_t shoup_small(uint32_t a, uint64_t w,
     return (uint64_t)a * w - (uint64_t)q * Prime<PI>::P;
 }
 
+/* Whether residues x_i in [0, p_i) of the three primes are those of one
+ * integer below p0, i.e. whether the Garner digits a1, a2 of
+ * x0 + a1*p0 + a2*p0*p1 are both 0, without computing them:
+ * a1 = (x1 - x0) * p0^-1 mod p1 is 0 iff x1 = x0 mod p1, and then
+ * a2 = (x2 - x0) * (p0 p1)^-1 mod p2 is 0 iff x2 = x0 mod p2.  As p0 < 2 p1
+ * and p0 < 2 p2, one conditional subtraction reduces x0 modulo each. */
+__host__ __device__ __forceinline__ bool crt_below_p0(uint64_t x0, uint64_t x1, uint64_t x2)
+{
+    static_assert(prime(0) < 2 * prime(1) && prime(0) < 2 * prime(2), "p0 < 2 p1 and p0 < 2 p2");
+    const uint64_t m1 = x0 >= prime(1) ? x0 - prime(1) : x0;
+    const uint64_t m2 = x0 >= prime(2) ? x0 - prime(2) : x0;
+    return (m1 == x1) & (m2 == x2);
+}
+
 /* Montgomery product a*b*2^-64 mod p for a*b < 2^64 * p; result in [0, p). */
 template <int PI>
 __host__ __device__ __forceinline__ uint64_t mont_mul(uint64_t a, uint64_t b)

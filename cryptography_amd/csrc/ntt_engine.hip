@@ -431,7 +431,6 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     ga.gen = ws.scratch;
     ga.blk = ws.scratch + (size_t)nprod * ga.blk_per_prod * 4 * CW_ROUNDS * 8;
     ga.flag = ws.flag ? ws.flag : g.flag;
-    ga.gc = g.gc;
     hipLaunchKernelGGL(k_garner_pack, dim3(ga.blk_per_prod, nprod), dim3(256), 0, st, ga);
     HIP_TRY(mark("k_garner_pack"));
     hipLaunchKernelGGL(k_carry_scan, dim3(nprod), dim3(1024), 0, st, ga.blk, ga.blk_per_prod, (uint8_t)0);

@@ -104,7 +104,6 @@ struct GarnerArgs {
     uint8_t *gen;               /* u64 generate mask per 64-limb round (8-byte aligned) */
     uint8_t *blk;               /* block aggregates, then block carry-ins */
     int *flag;                  /* integrity flag: set on a corrupted product */
-    GarnerConsts gc;
 };
 
 __global__ void k_twiddle_gen(TwPair *tw, uint64_t r2_0, uint64_t r2_1, uint64_t r2_2);

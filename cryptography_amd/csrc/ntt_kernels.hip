@@ -86,28 +86,19 @@ __global__ void __launch_bounds__(256) k_twiddle_gen(TwPair *tw, uint64_t r2_0, 
 /* Garner CRT + 16-bit repack + carry lookahead                               */
 /* ------------------------------------------------------------------------ */
 
-/* x = a0 + a1*p0 + a2*p0*p1 in the reference's order (garner.hip:135-173);
- * the product coefficients are < L*(2^16-1)^2 < 2^62 < p0, so a1 = a2 = 0 for
- * an intact product and the value is a0.  Any nonzero a1/a2 is a corrupted
- * product and raises the integrity flag. */
-__device__ __forceinline__ uint64_t garner3(uint64_t r0, uint64_t r1, uint64_t r2, const GarnerConsts &gc, int &bad)
+/* x = a0 + a1*p0 + a2*p0*p1 in the reference's mixed-radix form
+ * (garner.hip:135-173); the product coefficients are < L*(2^16-1)^2 < 2^62
+ * < p0, so a1 = a2 = 0 for an intact product and the value is a0 = x0.  Any
+ * nonzero a1/a2 is a corrupted product and raises the integrity flag; the
+ * test that both are 0 needs no product (crt_below_p0). */
+__device__ __forceinline__ uint64_t garner3(uint64_t r0, uint64_t r1, uint64_t r2, int &bad)
 {
-    const uint64_t p1 = Prime<1>::P, p2 = Prime<2>::P;
     /* inverse-pass output in [0, 4p) */
-    uint64_t x0 = reduce1<0>(reduce2<0>(r0));
-    uint64_t x1 = reduce1<1>(reduce2<1>(r1));
-    uint64_t x2 = reduce1<2>(reduce2<2>(r2));
-    uint64_t a0 = x0;
-    uint64_t a0m1 = a0 >= p1 ? a0 - p1 : a0;            /* a0 < p0 < 2*p1 */
-    uint64_t t1 = x1 >= a0m1 ? x1 - a0m1 : x1 + p1 - a0m1;
-    uint64_t a1 = reduce1<1>(shoup_lazy<1>(t1, gc.inv01, gc.inv01p));
-    uint64_t a0m2 = a0 >= p2 ? a0 - p2 : a0;
-    uint64_t b2 = a0m2 + reduce1<2>(shoup_lazy<2>(a1, gc.p0m2, gc.p0m2p));
-    b2 = b2 >= p2 ? b2 - p2 : b2;
-    uint64_t t2 = x2 >= b2 ? x2 - b2 : x2 + p2 - b2;
-    uint64_t a2 = reduce1<2>(shoup_lazy<2>(t2, gc.inv012, gc.inv012p));
-    bad |= (a1 | a2) != 0;
-    return a0;
+    const uint64_t x0 = reduce1<0>(reduce2<0>(r0));
+    const uint64_t x1 = reduce1<1>(reduce2<1>(r1));
+    const uint64_t x2 = reduce1<2>(reduce2<2>(r2));
+    bad |= !crt_below_p0(x0, x1, x2);
+    return x0;
 }
 
 /* Inclusive scan of 256 (or 1024) cells in LDS, lower thread index first. */
@@ -146,7 +137,6 @@ __device__ __forceinline__ void limb_sum(const GarnerArgs &ga, size_t off, size_
     if (k0 >= n_coef) return;
     const size_t at = res_at(off + k0, ga.pad_log);
     const uint64_t *r0 = ga.res + at, *r1 = ga.res + ga.len + at, *r2 = ga.res + 2 * ga.len + at;
-    const GarnerConsts &gc = ga.gc;
     uint64_t v0[4], v1[4], v2[4];
     if (k0 + 4 <= n_coef) {
         const ulonglong2 *p0 = (const ulonglong2 *)r0, *p1 = (const ulonglong2 *)r1,
@@ -165,7 +155,7 @@ __device__ __forceinline__ void limb_sum(const GarnerArgs &ga, size_t off, size_
     }
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        uint64_t X = garner3(v0[t], v1[t], v2[t], gc, bad);
+        uint64_t X = garner3(v0[t], v1[t], v2[t], bad);
         if (t == 0) {
             lo = X;
         } else {

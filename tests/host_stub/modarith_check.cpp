@@ -3,8 +3,9 @@
  * against 128-bit integer arithmetic, for the three NTT primes: the value and
  * the documented output range of the Montgomery product, the Shoup product
  * (and its small-operand form), both butterflies (the inverse one also with a
- * negated twiddle), and the compile-time roots.  Prints "bad N" and
- * exits non-zero on any mismatch.
+ * negated twiddle), the compile-time roots, and the CRT test that three
+ * residues are those of one integer below p0 (against Garner's digits).
+ * Prints "bad N" and exits non-zero on any mismatch.
  */
 #include "../../cryptography_amd/csrc/modarith.h"
 
@@ -107,12 +108,54 @@ static void check(long iters)
     expect(T5::w == powmod(T::w, 5, p) && T5::wp == (uint64_t)(((u128)T5::w << 64) / p), "root power", PI, T5::w, 0);
 }
 
+/* crt_below_p0 against the textbook Garner digits (128-bit arithmetic):
+ * residues of X are those of an integer below p0 iff a1 = a2 = 0 in
+ * X = x0 + a1 p0 + a2 p0 p1.  Integers below p0 (also in [p2, p0), where
+ * x0 must be reduced before the comparison), integers from p0 up, and
+ * random tuples. */
+static void check_crt(long iters)
+{
+    const uint64_t p0 = prime(0), p1 = prime(1), p2 = prime(2);
+    const uint64_t i01 = powmod(p0 % p1, p1 - 2, p1);
+    const uint64_t i012 = (uint64_t)((u128)powmod(p0 % p2, p2 - 2, p2) * powmod(p1 % p2, p2 - 2, p2) % p2);
+    auto digits_zero = [&](uint64_t x0, uint64_t x1, uint64_t x2) {
+        const uint64_t a1 = (uint64_t)((u128)((x1 + p1 - x0 % p1) % p1) * i01 % p1);
+        const uint64_t b = (uint64_t)(((u128)x0 + (u128)a1 * p0) % p2);
+        const uint64_t a2 = (uint64_t)((u128)((x2 + p2 - b) % p2) * i012 % p2);
+        return a1 == 0 && a2 == 0;
+    };
+    /* residues of X */
+    auto res = [&](u128 X, uint64_t &x0, uint64_t &x1, uint64_t &x2) {
+        x0 = (uint64_t)(X % p0);
+        x1 = (uint64_t)(X % p1);
+        x2 = (uint64_t)(X % p2);
+    };
+    auto one = [&](u128 X, bool below) {
+        uint64_t x0, x1, x2;
+        res(X, x0, x1, x2);
+        const bool got = crt_below_p0(x0, x1, x2);
+        expect(got == digits_zero(x0, x1, x2), "crt vs Garner digits", 0, (uint64_t)(X >> 64), (uint64_t)X);
+        expect(got == below, "crt below p0", 0, (uint64_t)(X >> 64), (uint64_t)X);
+    };
+    const u128 edges[] = {0, 1, p2 - 1, p2, p2 + 1, p1 - 1, p1, p1 + 1, p0 - 1, p0, p0 + 1, 2 * (u128)p0,
+                          (u128)p0 * p1, (u128)p0 * p1 - 1, (u128)p0 * p1 + p0 - 1, ~(u128)0};
+    for (u128 X : edges) one(X, X < p0);
+    for (long i = 0; i < iters; i++) {
+        one(rnd() % p0, true);
+        one(p2 + rnd() % (p0 - p2), true);
+        one(p0 + (((u128)rnd() << 64) | rnd()) % ((~(u128)0) - p0), false);
+        const uint64_t x0 = rnd() % p0, x1 = rnd() % p1, x2 = rnd() % p2;
+        expect(crt_below_p0(x0, x1, x2) == digits_zero(x0, x1, x2), "crt random tuple", 0, x0, x1);
+    }
+}
+
 int main(int argc, char **argv)
 {
     long iters = argc > 1 ? atol(argv[1]) : 2000000;
     check<0>(iters);
     check<1>(iters);
     check<2>(iters);
+    check_crt(iters / 8);
     printf("bad %d\n", bad);
     return bad != 0;
 }
