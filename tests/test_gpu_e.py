@@ -26,6 +26,14 @@ def test_e_matches_reference_output(d):
     assert st["n_terms"] == O.orc().orc_split_terms_needed(d + 20)
 
 
+@pytest.mark.parametrize("d", [1, 2, 3, 19, 20, 21, 49])
+def test_e_short_outputs_are_prefixes(d):
+    """d digits ("2." and d - 1 truncated decimals) are the first d + 1 characters of the
+    reference's 50-digit output: the string's length and end at the smallest sizes."""
+    s, _ = EN.compute_e(d)
+    assert s == REF["50"]["e"][:d + 1]
+
+
 def test_e_100k_matches_reference_hash():
     s, _ = EN.compute_e(100000)
     assert len(s) == REF["100000"]["len"]
