@@ -64,6 +64,28 @@ def test_every_transform_length(log_len):
         _check_mod(a, a, s)
 
 
+@pytest.mark.parametrize("log_len,cap", [(13, 4), (16, 4), (18, 6), (19, 5), (20, 4), (22, 5), (24, 4)])
+def test_pass_cap_plans(log_len, cap):
+    """A pass cap below the default (ntt_engine_set_pass_cap, or a loaded tuning
+    file) splits the transform into more, smaller passes: 3 to 6 passes of 2^3
+    to 2^6 points here, with several middle forward and inverse passes.
+    Product and square exact against GMP; the cap is restored afterwards."""
+    na, nb = _sizes(log_len)
+    a = O.splitmix(3000 + log_len, na)
+    b = O.splitmix(4000 + log_len, nb)
+    E.init()
+    L = E.lib()
+    E.engine.check(L.ntt_engine_set_pass_cap(log_len, cap))
+    try:
+        assert L.ntt_engine_get_pass_cap(log_len) == cap
+        c = E.mul(a, b)
+        s = E.sqr(a)
+    finally:
+        E.engine.check(L.ntt_engine_set_pass_cap(log_len, 0))
+    assert np.array_equal(c, O.gmp_mul(a, b))
+    assert np.array_equal(s, O.gmp_mul(a, a))
+
+
 def test_headline_2p30_bit_product_and_square():
     """2^30-bit x 2^30-bit (top bits set), L = 2^27: the product and the
     square modulo three primes, on device-resident operands."""
