@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--e-digits", type=int, default=10 ** 8)
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps of the product only (for rocprofv3), print nothing else")
+    ap.add_argument("--profile-op", choices=("mul", "sqr"), default="mul",
+                    help="with --profile-only: the product (mul) or the square of the first operand (sqr)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy (torch) to calibrate rocprofv3 byte counters")
     return ap.parse_args()
@@ -167,9 +169,12 @@ def headline(args, E, O):
         raise SystemExit("bench.py: GPU product differs from GMP")
     if args.profile_only:
         for _ in range(args.steps):
-            E.mul_device(dc, da, na, db, nb, sync=False)
+            if args.profile_op == "sqr":
+                E.sqr_device(dc, da, na, sync=False)
+            else:
+                E.mul_device(dc, da, na, db, nb, sync=False)
         E.engine.check(L.ntt_engine_sync())
-        return {"profile_only": True, "steps": args.steps, "transform_len": Llen}
+        return {"profile_only": True, "op": args.profile_op, "steps": args.steps, "transform_len": Llen}
     E.engine.check(L.ntt_engine_sync())
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -286,20 +291,11 @@ def workload_mul22(E, O):
 
 def workload_batch(E, O):
     from cryptography_amd import engine as EN
-    rng = np.random.default_rng(4096)
+    import _workloads as W
     n = 4096
-    bits = np.floor(2.0 ** (12 + 10 * rng.random((n, 2)))).astype(np.int64)
+    bits = W.config4_sizes(n)
     la, lb = (bits[:, 0] + 63) // 64, (bits[:, 1] + 63) // 64
-    pool = EN.DevicePool(int((la + lb).sum() * 2) + 16 * n)
-    items, host = [], []
-    for i in range(n):
-        a = O.random_operand(10_000 + 2 * i, int(bits[i, 0]))
-        b = O.random_operand(10_000 + 2 * i + 1, int(bits[i, 1]))
-        oa, ob, oc = pool.take(len(a)), pool.take(len(b)), pool.take(len(a) + len(b))
-        pool.upload(oa, a)
-        pool.upload(ob, b)
-        items.append((pool.ptr(oa), len(a), pool.ptr(ob), len(b), pool.ptr(oc), len(a) + len(b), None, 0))
-        host.append((a, b, oc))
+    pool, items, host = W.config4_batch(EN, n)
     descs = EN.make_descs(items)
     t0 = time.perf_counter()
     g = EN.BatchGraph(descs)

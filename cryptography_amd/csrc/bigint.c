@@ -14,7 +14,8 @@
 
 typedef unsigned __int128 u128;
 
-/* Managed allocation lives in the HIP part of the library (ntt_glue.hip). */
+/* Managed allocation lives in the HIP part of the library (ntt_engine.hip:
+ * hipMallocManaged, cf. ntt/lib/arith/bigint_hip_alloc.hip). */
 extern limb_t *ntt_managed_alloc_limbs(int cap);
 extern void ntt_managed_free_limbs(limb_t *p);
 

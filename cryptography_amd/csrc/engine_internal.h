@@ -65,6 +65,10 @@ struct Engine {
 
 extern Engine g;
 extern std::recursive_mutex g_lock;
+/* Incremented by every successful engine initialisation.  Handles that
+ * captured the engine's tables (batch graphs, product trees) record it and
+ * refuse to run after a teardown and re-initialisation freed those tables. */
+extern unsigned long g_generation;
 
 /* Device workspace of one executor (the engine's own, or a graph's). */
 struct Workspace {
@@ -186,7 +190,9 @@ unsigned par_chunks(size_t n, F &&f, size_t min_chunk = (size_t)1 << 15)
 }
 int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hipStream_t st);
 int sync_check();
-int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int square);
+/* force_ntt: the CRT-NTT path whatever the operand sizes (no schoolbook). */
+int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int square,
+                int force_ntt = 0);
 int copy_h2d(void *dst, const void *src, size_t bytes);
 int copy_d2h(void *dst, const void *src, size_t bytes);
 

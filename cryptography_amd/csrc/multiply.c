@@ -48,7 +48,8 @@ void ntt_mul_teardown(void) { ntt_mul_teardown_impl(); }
 unsigned long ntt_get_gpu_dispatch_count(void) { return ntt_engine_dispatch_count(); }
 
 /* Product into a temporary, then into c: alias-safe and c's storage is only
- * resized after the operands have been read. */
+ * resized after the operands have been read.  A managed c that is too short
+ * aborts in bigint_resize, as in the reference (bigint.c:46-60). */
 static void mul_common(BigInt *c, const BigInt *a, const BigInt *b, int square, int force_ntt, const char *what)
 {
     ensure_init(what);
@@ -64,10 +65,11 @@ static void mul_common(BigInt *c, const BigInt *a, const BigInt *b, int square, 
         fprintf(stderr, "%s: out of memory\n", what);
         exit(1);
     }
-    int saved = ntt_engine_get_threshold();
-    if (force_ntt) ntt_engine_set_threshold(0);
-    int rc = square ? ntt_sqr_limbs(tmp, a->limbs, na) : ntt_mul_limbs(tmp, a->limbs, na, b->limbs, nb);
-    if (force_ntt) ntt_engine_set_threshold(saved);
+    /* the forced NTT path is a flag of the call, not a change of the shared
+     * threshold, so concurrent bigint_mul callers never see it */
+    int rc = square      ? ntt_sqr_limbs(tmp, a->limbs, na)
+             : force_ntt ? ntt_mul_limbs_ntt(tmp, a->limbs, na, b->limbs, nb)
+                         : ntt_mul_limbs(tmp, a->limbs, na, b->limbs, nb);
     if (rc) {
         free(tmp);
         fatal(what, rc);

@@ -28,6 +28,7 @@ namespace ntte {
 
 Engine g;
 std::recursive_mutex g_lock;
+unsigned long g_generation = 0;
 
 Workspace g_ws;
 
@@ -116,6 +117,7 @@ int engine_init(int device)
     g.gc.inv012 = h_mulmod(h_inv(p0 % p2, p2), h_inv(p1 % p2, p2), p2);
     g.gc.inv012p = h_shoup(g.gc.inv012, p2);
     g.init = 1;
+    g_generation++;
     return NTT_OK;
 }
 
@@ -472,7 +474,8 @@ int check_desc(const MulDesc &d)
 }
 
 /* Enqueue c = a*b (or a^2) on the engine stream.  Device pointers. */
-int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int square)
+int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int square,
+                int force_ntt)
 {
     if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
     if ((na && !d_a) || (nb && !d_b) || ((na + nb) && !d_c)) return fail(NTT_ERR_ARG, "null operand pointer");
@@ -497,7 +500,7 @@ int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d
     one.na = (uint32_t)na;
     one.nb = (uint32_t)nb;
     one.n_out = (uint32_t)nout;
-    if (use_schoolbook(na, nb, nout)) return enqueue_schoolbook(nullptr, one, 1, g.stream, nullptr);
+    if (!force_ntt && use_schoolbook(na, nb, nout)) return enqueue_schoolbook(nullptr, one, 1, g.stream, nullptr);
     const int logL = log_len(na, nb);
     size_t rw, sb;
     class_needs(logL, 1, (uint32_t)nout, square, rw, sb);
@@ -750,7 +753,7 @@ static int d2h_staged(void *dst, const void *src, size_t bytes)
     return NTT_OK;
 }
 
-int host_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, int square)
+int host_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb, int square, int force_ntt)
 {
     if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
     if ((na && !a) || (nb && !b) || ((na + nb) && !c)) return fail(NTT_ERR_ARG, "null operand pointer");
@@ -762,7 +765,7 @@ int host_mul(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_
     if (rc) return rc;
     if (na) rc = h2d_staged(g.d_a, a, na * 8);
     if (!rc && !square && nb) rc = h2d_staged(g.d_b, b, nb * 8);
-    if (!rc) rc = enqueue_mul(g.d_c, g.d_a, na, square ? g.d_a : g.d_b, nb, square);
+    if (!rc) rc = enqueue_mul(g.d_c, g.d_a, na, square ? g.d_a : g.d_b, nb, square, force_ntt);
     if (!rc) rc = d2h_staged(c, g.d_c, nout * 8);
     if (rc) {
         (void)hipStreamSynchronize(g.stream);
@@ -798,13 +801,19 @@ int ntt_engine_log_len(size_t na, size_t nb) { return log_len(na, nb); }
 int ntt_mul_limbs(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb)
 {
     std::lock_guard<std::recursive_mutex> lk(g_lock);
-    return host_mul(c, a, na, b, nb, 0);
+    return host_mul(c, a, na, b, nb, 0, 0);
+}
+
+int ntt_mul_limbs_ntt(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    return host_mul(c, a, na, b, nb, 0, 1);
 }
 
 int ntt_sqr_limbs(uint64_t *c, const uint64_t *a, size_t na)
 {
     std::lock_guard<std::recursive_mutex> lk(g_lock);
-    return host_mul(c, a, na, a, na, 1);
+    return host_mul(c, a, na, a, na, 1, 0);
 }
 
 int ntt_mul_device_async(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb)
@@ -884,6 +893,7 @@ struct ntt_batch_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     size_t n = 0;
+    unsigned long gen = 0;      /* engine generation the graph was captured in */
 };
 
 void ntt_batch_graph_destroy(ntt_batch_graph *gr)
@@ -907,6 +917,7 @@ int ntt_batch_graph_create(const ntt_mul_desc *descs, size_t n, ntt_batch_graph 
     if (n == 0 || !descs) return fail(NTT_ERR_ARG, "empty batch");
     ntt_batch_graph *gr = new ntt_batch_graph();
     gr->n = n;
+    gr->gen = g_generation;
     int rc = plan_batch(descs, n, gr->bp);
     if (!rc) rc = ws_reserve(gr->ws, gr->bp.res_words ? gr->bp.res_words : 1, gr->bp.scratch ? gr->bp.scratch : 64);
     if (!rc && hipMalloc((void **)&gr->d_desc, n * sizeof(MulDesc)) != hipSuccess)
@@ -947,6 +958,8 @@ int ntt_batch_graph_launch(ntt_batch_graph *gr)
     std::lock_guard<std::recursive_mutex> lk(g_lock);
     if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
     if (!gr || !gr->exec) return fail(NTT_ERR_ARG, "null graph");
+    if (gr->gen != g_generation)
+        return fail(NTT_ERR_NOT_INIT, "batch graph captured before the engine was torn down and re-initialised");
     g.dispatches += gr->n;
     HIP_TRY(hipGraphLaunch(gr->exec, g.stream));
     return NTT_OK;

@@ -109,15 +109,14 @@ extern "C" int ntt_serve_replay(const ntt_mul_desc *reqs, const double *arrival,
     if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
     if (!reqs || !arrival || !t_done || n == 0 || nslots < 1 || nslots > 16)
         return fail(NTT_ERR_ARG, "bad serve arguments");
-    for (size_t i = 0; i < n; i++) {
-        MulDesc d;
-        memcpy(&d, &reqs[i], sizeof d);
-        if (req_mem(reqs[i]) > hbm_budget / nslots)
+    /* a slot's workspace is words * 8 bytes: the same bound admits a request
+     * here and in the loop below, so every request that passes can run */
+    const size_t words = hbm_budget / nslots / 8;
+    for (size_t i = 0; i < n; i++)
+        if (req_mem(reqs[i]) > words * 8)
             return fail(NTT_ERR_TOO_LARGE, "request %zu needs %zu bytes, more than a slot's share of the budget",
                         i, req_mem(reqs[i]));
-    }
     Server sv;
-    size_t words = hbm_budget / nslots / 8;
     int rc = sv.init(hbm_budget, nslots, words);
     if (rc) {
         sv.release();

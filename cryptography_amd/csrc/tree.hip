@@ -114,6 +114,7 @@ struct ntt_split_tree {
     hipGraphExec_t exec = nullptr;
     size_t n_products = 0;
     int per_product = 0;
+    unsigned long gen = 0;      /* engine generation the tree was planned in */
 };
 
 static void tree_free(ntt_split_tree *t)
@@ -175,6 +176,7 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
     t->a = a;
     t->b = b;
     t->per_product = use_graph == 2;
+    t->gen = g_generation;
     /* The tree in one array, depth by depth, left to right: node i covers
      * [lo[i], lo[i] + sz[i]); an internal node's children are kid[i] and
      * kid[i] + 1.  Then three passes over all nodes in parallel chunks:
@@ -336,6 +338,8 @@ int ntt_split_tree_run(ntt_split_tree *t)
     std::lock_guard<std::recursive_mutex> lk(g_lock);
     if (!g.init) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
     if (!t) return fail(NTT_ERR_ARG, "null tree");
+    if (t->gen != g_generation)
+        return fail(NTT_ERR_NOT_INIT, "product tree made before the engine was torn down and re-initialised");
     if (t->exec) {
         g.dispatches += t->n_products;
         HIP_TRY(hipGraphLaunch(t->exec, g.stream));

@@ -22,6 +22,15 @@ typedef uint64_t limb_t;
 #define LIMB_MASK (~(uint64_t)0)
 #define LIMB_BYTES ((int)sizeof(limb_t))
 
+/* bigint.h:52-66: the full 128-bit product of two limbs, *hi:*lo = a*b.  At
+ * 64-bit limbs the compiler's 128-bit multiply gives it directly. */
+static inline void mul_limb(limb_t a, limb_t b, limb_t *lo, limb_t *hi)
+{
+    const unsigned __int128 p = (unsigned __int128)a * b;
+    *lo = (limb_t)p;
+    *hi = (limb_t)(p >> 64);
+}
+
 /* bigint.h:69-74 */
 typedef struct {
     limb_t *limbs;

@@ -49,6 +49,10 @@ int ntt_engine_log_len(size_t na, size_t nb);
 /* Host-buffer multiply / square: c[0..na+nb) = a*b.  Synchronous. */
 int ntt_mul_limbs(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb);
 int ntt_sqr_limbs(uint64_t *c, const uint64_t *a, size_t na);
+/* The same product on the CRT-NTT path whatever the operand sizes (the
+ * status-returning form of ntt_bigint_mul, multiply.h:46): the threshold is
+ * neither read nor changed, so concurrent callers of bigint_mul keep theirs. */
+int ntt_mul_limbs_ntt(uint64_t *c, const uint64_t *a, size_t na, const uint64_t *b, size_t nb);
 
 /* Device-resident multiply / square on the engine stream.  d_c, d_a, d_b are
  * device pointers; d_c must not alias d_a or d_b.  Synchronous: returns after

@@ -15,7 +15,7 @@ typedef unsigned __int128 u128;
 
 /* Three primes c*2^k+1 < 2^62 (so lazy values in [0,4p) fit a u64 word).
  * Generators were found by testing g^((p-1)/q) != 1 for every prime q | p-1;
- * tests/test_oracle_primes.py re-proves primality, the factorisation of p-1
+ * tests/test_oracle.py (test_primes_form_and_generators) re-proves primality, the factorisation of p-1
  * and the generator property from scratch. */
 static const uint64_t PR[ORC_NPRIMES]  = {0x3fffc00000000001ull, 0x3fffbe0000000001ull,
                                           0x3fff840000000001ull};

@@ -2,6 +2,7 @@
 
 Each product of a batch must equal the same product computed alone, and GMP.
 """
+import ctypes
 import random
 
 import numpy as np
@@ -191,4 +192,39 @@ def test_batch_table_passes_many_products(count, na, nb):
             g.close()
     finally:
         EN.set_xtab(old)
+        pool.close()
+
+
+def test_handles_refuse_to_run_after_reinit():
+    """A batch graph or product tree captured before ntt_engine_teardown holds
+    the old engine's tables: after a re-initialisation it must return
+    NTT_ERR_NOT_INIT instead of replaying (its kernels are never launched),
+    while handles made after it run normally."""
+    E.init()
+    L = E.lib()
+    pool = EN.DevicePool(100000)
+    try:
+        descs, host = build_batch(pool, [(300, 200, 0), (2000, 2100, 0)], seed=9)
+        old = EN.BatchGraph(descs)
+        tree_h = ctypes.c_void_p()
+        L.ntt_split_tree_create.argtypes = [ctypes.c_long, ctypes.c_long, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_void_p)]
+        L.ntt_split_tree_run.argtypes = [ctypes.c_void_p]
+        L.ntt_split_tree_destroy.argtypes = [ctypes.c_void_p]
+        EN.check(L.ntt_split_tree_create(1, 3001, 1, ctypes.byref(tree_h)))
+        L.ntt_engine_teardown()
+        E.init()
+        with pytest.raises(E.NttError) as ei:
+            old.launch()
+        assert ei.value.code == 1
+        assert L.ntt_split_tree_run(tree_h) == 1
+        old.close()
+        L.ntt_split_tree_destroy(tree_h)
+        new = EN.BatchGraph(descs)
+        new.launch()
+        EN.sync()
+        for (a, b, add, oc, n_out) in host:
+            assert np.array_equal(pool.download(oc, n_out), expected(a, b, add, n_out))
+        new.close()
+    finally:
         pool.close()

@@ -98,3 +98,22 @@ def test_headline_2p30_bit_product_and_square():
     _check_mod(a, b, dc.download())
     E.sqr_device(dc, da, len(a))
     _check_mod(a, a, dc.download())
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("log_len", [28, 29, 30])
+def test_longest_transform_lengths(log_len):
+    """The lengths above the headline, up to the largest the engine accepts
+    (2^30 coefficients: 48 GiB of residue rows): unequal device-resident
+    operands, the product and a square modulo three primes."""
+    na, nb = _sizes(log_len)
+    assert E.log_len(na, nb) == log_len
+    print(f"log_len {log_len}: {na} x {nb} limbs", flush=True)
+    a = O.splitmix(5000 + log_len, na)
+    b = O.splitmix(6000 + log_len, nb)
+    with E.DeviceBuffer.from_array(a) as da, E.DeviceBuffer.from_array(b) as db, \
+            E.DeviceBuffer(na + nb) as dc:
+        E.mul_device(dc, da, na, db, nb)
+        _check_mod(a, b, dc.download())
+        E.sqr_device(dc, da, na)
+        _check_mod(a, a, dc.download(2 * na))
