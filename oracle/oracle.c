@@ -107,6 +107,96 @@ static void ct_dit(uint64_t *a, int log_n, uint64_t w, uint64_t p)
     free(twp);
 }
 
+/* The same CT-DIT transform on all host cores, for the CPU baseline at full
+ * size (ntt_cpu.c:158-202's arithmetic unchanged: bit reversal, then for
+ * len = 2, 4, .., n the butterflies (u + t, u - t), t = a[k+j+half] *
+ * w^(j n/len) by Shoup with a full reduction).  Differences are of schedule
+ * only: the twiddle table w^k (k < n/2) with its Shoup partners is built once
+ * by the caller (tw_build) and reused across transforms; the stages with
+ * len <= ORC_BLK run block by block while a block sits in cache; every larger
+ * stage splits its butterflies over the threads. */
+#define ORC_BLK ((size_t)1 << 14)
+
+typedef struct { uint64_t *w, *wp; size_t half; } TwTab;
+
+static int tw_build(TwTab *t, uint64_t w, uint64_t p, int log_n, int nt)
+{
+    const size_t half = ((size_t)1 << log_n) >> 1;
+    t->half = half;
+    t->w = (uint64_t *)malloc((half ? half : 1) * sizeof(uint64_t));
+    t->wp = (uint64_t *)malloc((half ? half : 1) * sizeof(uint64_t));
+    if (!t->w || !t->wp) { free(t->w); free(t->wp); return -1; }
+    const uint64_t wsh = orc_shoup_pre(w, p);
+    const size_t chunk = 1 << 16;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+    for (size_t c0 = 0; c0 < half; c0 += chunk) {
+        uint64_t x = orc_powmod(w, c0, p);           /* w^c0, then successive products */
+        const size_t c1 = c0 + chunk < half ? c0 + chunk : half;
+        for (size_t k = c0; k < c1; k++) {
+            t->w[k] = x;
+            t->wp[k] = orc_shoup_pre(x, p);
+            x = orc_shoup_mul(x, w, wsh, p);
+        }
+    }
+    (void)nt;
+    return 0;
+}
+
+static void tw_free(TwTab *t) { free(t->w); free(t->wp); }
+
+static inline void dit_stage_range(uint64_t *a, size_t k0, size_t k1, size_t len, size_t stride, const TwTab *t,
+                                   uint64_t p)
+{
+    const size_t half = len >> 1;
+    for (size_t k = k0; k < k1; k += len)
+        for (size_t j = 0; j < half; j++) {
+            const uint64_t u = a[k + j];
+            const uint64_t v = orc_shoup_mul(a[k + j + half], t->w[j * stride], t->wp[j * stride], p);
+            a[k + j] = orc_addmod(u, v, p);
+            a[k + j + half] = orc_submod(u, v, p);
+        }
+}
+
+static void ct_dit_par(uint64_t *a, int log_n, const TwTab *t, uint64_t p, int nt)
+{
+    const size_t n = (size_t)1 << log_n;
+    if (n < 2) return;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+    for (size_t i = 0; i < n; i++) {            /* each pair swapped once, by its smaller index */
+        size_t r = 0, x = i;
+        for (int b = 0; b < log_n; b++) { r = (r << 1) | (x & 1); x >>= 1; }
+        if (i < r) { uint64_t v = a[i]; a[i] = a[r]; a[r] = v; }
+    }
+    const size_t blk = n < ORC_BLK ? n : ORC_BLK;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+    for (size_t b0 = 0; b0 < n; b0 += blk)
+        for (size_t len = 2; len <= blk; len <<= 1) dit_stage_range(a, b0, b0 + blk, len, n / len, t, p);
+    for (size_t len = blk << 1; len <= n; len <<= 1) {
+        const size_t half = len >> 1, stride = n / len;
+        /* butterflies (k, j) over all groups k and offsets j, in pieces */
+        const size_t piece = half < 4096 ? half : 4096;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+        for (size_t q = 0; q < n / 2; q += piece) {
+            const size_t k = (q / half) * len, j0 = q % half;
+            for (size_t j = j0; j < j0 + piece; j++) {
+                const uint64_t u = a[k + j];
+                const uint64_t v = orc_shoup_mul(a[k + j + half], t->w[j * stride], t->wp[j * stride], p);
+                a[k + j] = orc_addmod(u, v, p);
+                a[k + j + half] = orc_submod(u, v, p);
+            }
+        }
+    }
+    (void)nt;
+}
+
 void orc_ntt_forward(uint64_t *a, int log_n, int pi)
 {
     ct_dit(a, log_n, orc_root(pi, log_n), PR[pi]);
@@ -137,11 +227,29 @@ void orc_dft_naive(const uint64_t *in, uint64_t *out, int n, uint64_t w, uint64_
  * garner_one): a0 = r0; a1 = (r1 - a0 mod p1) * p0^-1 mod p1;
  * a2 = (r2 - (a0 + a1*p0) mod p2) * (p0*p1)^-1 mod p2; x = a0 + a1*p0 + a2*p0*p1.
  * The reference runs the same recurrence over four residues into a U256. */
+typedef struct { uint64_t inv01, inv02, inv12; } GarnerK;
+
+static GarnerK garner_consts(void)
+{
+    GarnerK k;
+    k.inv01 = orc_invmod(PR[0] % PR[1], PR[1]);
+    k.inv02 = orc_invmod(PR[0] % PR[2], PR[2]);
+    k.inv12 = orc_invmod(PR[1] % PR[2], PR[2]);
+    return k;
+}
+
+static void garner3_k(uint64_t r0, uint64_t r1, uint64_t r2, const GarnerK *K, uint64_t out[3]);
+
 void orc_garner3(uint64_t r0, uint64_t r1, uint64_t r2, uint64_t out[3])
 {
+    const GarnerK K = garner_consts();
+    garner3_k(r0, r1, r2, &K, out);
+}
+
+static void garner3_k(uint64_t r0, uint64_t r1, uint64_t r2, const GarnerK *K, uint64_t out[3])
+{
     const uint64_t p0 = PR[0], p1 = PR[1], p2 = PR[2];
-    uint64_t inv01 = orc_invmod(p0 % p1, p1);
-    uint64_t inv02 = orc_invmod(p0 % p2, p2), inv12 = orc_invmod(p1 % p2, p2);
+    const uint64_t inv01 = K->inv01, inv02 = K->inv02, inv12 = K->inv12;
     uint64_t a0 = r0;
     uint64_t a1 = orc_mulmod(orc_submod(r1, a0 % p1, p1), inv01, p1);
     uint64_t base2 = orc_addmod(a0 % p2, orc_mulmod(a1, p0 % p2, p2), p2);
@@ -243,6 +351,8 @@ int orc_log_len(size_t na, size_t nb)
     return l;
 }
 
+static int orc_mul_finish(uint64_t **ra, size_t na, size_t nb, size_t n, uint64_t *out);
+
 /* Pipeline order of ntt_bigint_mul (ntt/lib/arith/ntt_mul.hip:995-1235):
  * split -> forward(a), forward(b) per prime -> pointwise -> inverse ->
  * Garner -> carry assembly.  The product has at most na+nb limbs. */
@@ -275,14 +385,88 @@ int orc_mul_ntt3(const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
         free(fb);
     }
     (void)n_threads;
+    return orc_mul_finish(ra, na, nb, n, out);
+}
+
+/* Garner and the carry of orc_mul_ntt3 over its three residue rows. */
+static int orc_mul_finish(uint64_t **ra, size_t na, size_t nb, size_t n, uint64_t *out)
+{
     size_t ncoef = 4 * (na + nb);
     if (ncoef > n) ncoef = n;
     uint64_t *X = (uint64_t *)malloc(3 * ncoef * sizeof(uint64_t));
-    for (size_t k = 0; k < ncoef; k++) orc_garner3(ra[0][k], ra[1][k], ra[2][k], X + 3 * k);
+    const GarnerK K = garner_consts();
+    for (size_t k = 0; k < ncoef; k++) garner3_k(ra[0][k], ra[1][k], ra[2][k], &K, X + 3 * k);
     uint64_t spill = orc_pack16(X, ncoef, out, na + nb);
     free(X);
     for (int i = 0; i < ORC_NPRIMES; i++) free(ra[i]);
     return spill ? -1 : 0;
+}
+
+/* orc_mul_ntt3 with every transform spread over n_threads host threads (the
+ * all-core CPU baseline): the primes one after the other, each prime's
+ * forward table built once for both operands, Garner and the split in
+ * parallel, the carry sequential.  Same products as orc_mul_ntt3. */
+int orc_mul_ntt3_par(const uint64_t *a, size_t na, const uint64_t *b, size_t nb, uint64_t *out, int n_threads)
+{
+    if (na == 0 || nb == 0) { memset(out, 0, (na + nb) * sizeof(uint64_t)); return 0; }
+    const int log_n = orc_log_len(na, nb);
+    if (log_n > 30) return -1;
+    const int nt = n_threads > 0 ? n_threads : 1;
+    const size_t n = (size_t)1 << log_n;
+    uint64_t *ra[ORC_NPRIMES] = {0};
+    uint64_t *fb = (uint64_t *)malloc(n * sizeof(uint64_t));
+    int ok = fb != NULL;
+    for (int i = 0; i < ORC_NPRIMES; i++) {
+        ra[i] = (uint64_t *)malloc(n * sizeof(uint64_t));
+        if (!ra[i]) ok = 0;
+    }
+    for (int i = 0; ok && i < ORC_NPRIMES; i++) {
+        const uint64_t p = PR[i], w = orc_root(i, log_n);
+        uint64_t *x = ra[i];
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+        for (size_t k = 0; k < n; k++) {
+            const size_t li = k >> 2;
+            x[k] = li < na ? (a[li] >> (16 * (k & 3))) & 0xffff : 0;
+            fb[k] = li < nb ? (b[li] >> (16 * (k & 3))) & 0xffff : 0;
+        }
+        TwTab t;
+        if (tw_build(&t, w, p, log_n, nt)) { ok = 0; break; }
+        ct_dit_par(x, log_n, &t, p, nt);
+        ct_dit_par(fb, log_n, &t, p, nt);
+        tw_free(&t);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+        for (size_t k = 0; k < n; k++) x[k] = orc_mulmod(x[k], fb[k], p);
+        if (tw_build(&t, orc_invmod(w, p), p, log_n, nt)) { ok = 0; break; }
+        ct_dit_par(x, log_n, &t, p, nt);
+        tw_free(&t);
+        const uint64_t ninv = orc_invmod(n % p, p), ninvp = orc_shoup_pre(ninv, p);
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+        for (size_t k = 0; k < n; k++) x[k] = orc_shoup_mul(x[k], ninv, ninvp, p);
+    }
+    free(fb);
+    int rc = -1;
+    if (ok) {
+        size_t ncoef = 4 * (na + nb);
+        if (ncoef > n) ncoef = n;
+        uint64_t *X = (uint64_t *)malloc(3 * ncoef * sizeof(uint64_t));
+        if (X) {
+            const GarnerK K = garner_consts();
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nt) schedule(static)
+#endif
+            for (size_t k = 0; k < ncoef; k++) garner3_k(ra[0][k], ra[1][k], ra[2][k], &K, X + 3 * k);
+            rc = orc_pack16(X, ncoef, out, na + nb) ? -1 : 0;
+            free(X);
+        }
+    }
+    for (int i = 0; i < ORC_NPRIMES; i++) free(ra[i]);
+    return rc;
 }
 
 /* ntt/lib/arith/multiply.c:35-67 (mul_schoolbook): per row a u128 carry runs

@@ -77,6 +77,10 @@ int      orc_pack16_cla(const uint64_t *X, size_t n_coeffs, uint64_t *out, size_
  * n_threads > 1 runs the per-prime transforms with OpenMP.  Returns 0 or -1. */
 int orc_mul_ntt3(const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
                  uint64_t *out, int n_threads);
+/* The same product with every transform spread over n_threads threads (the
+ * all-core CPU baseline; identical results). */
+int orc_mul_ntt3_par(const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
+                     uint64_t *out, int n_threads);
 /* O(na*nb) schoolbook multiply with a u128 carry; out has na+nb limbs. */
 void orc_mul_schoolbook(const uint64_t *a, size_t na, const uint64_t *b, size_t nb,
                         uint64_t *out);

@@ -61,6 +61,8 @@ def orc():
         L.orc_pack16_cla.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t]
         L.orc_mul_ntt3.restype = ctypes.c_int
         L.orc_mul_ntt3.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p, ctypes.c_int]
+        L.orc_mul_ntt3_par.restype = ctypes.c_int
+        L.orc_mul_ntt3_par.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p, ctypes.c_int]
         L.orc_mul_schoolbook.argtypes = [_u64p, ctypes.c_size_t, _u64p, ctypes.c_size_t, _u64p]
         L.orc_log_len.restype = ctypes.c_int
         L.orc_log_len.argtypes = [ctypes.c_size_t, ctypes.c_size_t]
@@ -150,6 +152,16 @@ def mul_ntt3(a, b, threads=1):
     rc = orc().orc_mul_ntt3(_ptr(a), len(a), _ptr(b), len(b), _ptr(out), threads)
     if rc != 0:
         raise RuntimeError("orc_mul_ntt3 failed")
+    return out
+
+
+def mul_ntt3_par(a, b, threads):
+    """The oracle's 3-prime NTT multiply with every transform on `threads` host threads."""
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = np.ascontiguousarray(b, dtype=np.uint64)
+    out = np.zeros(len(a) + len(b), dtype=np.uint64)
+    if orc().orc_mul_ntt3_par(_ptr(a), len(a), _ptr(b), len(b), _ptr(out), threads) != 0:
+        raise RuntimeError("orc_mul_ntt3_par failed")
     return out
 
 

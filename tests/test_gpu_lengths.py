@@ -111,9 +111,10 @@ def test_longest_transform_lengths(log_len):
     print(f"log_len {log_len}: {na} x {nb} limbs", flush=True)
     a = O.splitmix(5000 + log_len, na)
     b = O.splitmix(6000 + log_len, nb)
+    # the square of a has 2 na > na + nb limbs: it gets its own output buffer
     with E.DeviceBuffer.from_array(a) as da, E.DeviceBuffer.from_array(b) as db, \
-            E.DeviceBuffer(na + nb) as dc:
+            E.DeviceBuffer(na + nb) as dc, E.DeviceBuffer(2 * na) as ds:
         E.mul_device(dc, da, na, db, nb)
         _check_mod(a, b, dc.download())
-        E.sqr_device(dc, da, na)
-        _check_mod(a, a, dc.download(2 * na))
+        E.sqr_device(ds, da, na)
+        _check_mod(a, a, ds.download())
