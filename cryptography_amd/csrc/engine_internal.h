@@ -50,6 +50,12 @@ struct Engine {
      * use_xtab = 0 keeps every pass on the in-round Montgomery chains */
     uint64_t *xtab[FT_MAX_LOGS + 1][NTT_XTAB_MAX_LOG + 1] = {};
     int use_xtab = 1;
+    /* set while a call that must not allocate runs: xtab_for then hands out
+     * only the tables that already exist (the passes fall back to chains) */
+    int no_alloc = 0;
+    /* device address of this device's copy of the no-allocation scratch
+     * array (hipGetSymbolAddress at init; the array's name is not it) */
+    uint64_t *noalloc_scratch = nullptr;
     unsigned long dispatches = 0;
     int timing = 0;
     /* timing marks: one event after each launch of the last multiply */

@@ -19,6 +19,7 @@
 #define NTT_TW_LOG 15                      /* two-level table: 2^15 entries each */
 #define NTT_TW_SIZE (1u << NTT_TW_LOG)
 #define NTT_MAX_LOG_LEN NTT_ROOT_LOG       /* longest transform: 2^30 coefficients */
+#define NTT_NOALLOC_MAX_LOG 19             /* longest transform of the no-allocation multiply */
 
 namespace nttc {
 

@@ -32,6 +32,19 @@ unsigned long g_generation = 0;
 
 Workspace g_ws;
 
+/* Scratch of the no-allocation multiply (ntt_mul_device_noalloc): the residue
+ * rows of one product whose transform has at most 2^NTT_NOALLOC_MAX_LOG
+ * coefficients (2 operands x 3 primes x 2^19 words, plus room for the row
+ * pads a tuned pass split may add: 24.8 MB), then its carry cells
+ * (generate masks and block cells of up to 2^17 limbs: 17 KB of 64 KiB).
+ * One copy per device, held from module load to process end.  Host code uses
+ * the address hipGetSymbolAddress returns (g.noalloc_scratch), never the
+ * array's name. */
+#define NOALLOC_ROW_WORDS (((size_t)1 << NTT_NOALLOC_MAX_LOG) + ((size_t)1 << (NTT_NOALLOC_MAX_LOG - 12)) * NTT_PAD_WORDS)
+#define NOALLOC_RES_WORDS ((size_t)2 * NTT_NPRIMES * NOALLOC_ROW_WORDS)
+#define NOALLOC_CARRY_BYTES ((size_t)64 << 10)
+static __device__ __align__(256) uint64_t scratch[NOALLOC_RES_WORDS + NOALLOC_CARRY_BYTES / 8];
+
 /* Record a timing event named for the launch that precedes it. */
 hipError_t mark(const char *name)
 {
@@ -94,6 +107,8 @@ int engine_init(int device)
     HIP_TRY(hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking));
     HIP_TRY(hipMalloc((void **)&g.tw, sizeof(TwPair) * NTT_NPRIMES * 2 * NTT_TW_SIZE));
     HIP_TRY(hipMalloc((void **)&g.flag, sizeof(int)));
+    /* this device's copy of the no-allocation scratch array */
+    HIP_TRY(hipGetSymbolAddress((void **)&g.noalloc_scratch, HIP_SYMBOL(scratch)));
     HIP_TRY(hipMemsetAsync(g.flag, 0, sizeof(int), g.stream));
     for (int i = 0; i < NTT_MAX_MARKS; i++) HIP_TRY(hipEventCreate(&g.ev[i]));
     uint64_t r2[3];
@@ -198,6 +213,7 @@ const uint64_t *xtab_for(int logS, int logT, hipStream_t st)
         return nullptr;
     uint64_t *&t = g.xtab[logS][logN];
     if (t) return t;
+    if (g.no_alloc) return nullptr;      /* the passes use the chains instead */
     /* made on the engine stream, which must not be recording a graph */
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone, cg = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
@@ -508,6 +524,52 @@ int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d
     if (rc) return rc;
     rc = enqueue_ntt_class(g_ws, nullptr, one, 1, logL, (uint32_t)nout, square, g.stream, 0, 0,
                            (uint32_t)std::max(na, nb));
+    g.timing_pending = g.timing;
+    return rc;
+}
+
+/* c = a*b on the engine stream with the static scratch as workspace: no
+ * device or host allocation (products up to 2^NTT_NOALLOC_MAX_LOG
+ * coefficients).  The caller holds g_lock. */
+int enqueue_mul_noalloc(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb)
+{
+    if (!g.init || !g.noalloc_scratch) return fail(NTT_ERR_NOT_INIT, "engine not initialised");
+    if ((na && !d_a) || (nb && !d_b) || ((na + nb) && !d_c)) return fail(NTT_ERR_ARG, "null operand pointer");
+    if (na > 0xffffffffu || nb > 0xffffffffu || log_len(na, nb) > NTT_NOALLOC_MAX_LOG)
+        return fail(NTT_ERR_TOO_LARGE,
+                    "product of %zu x %zu limbs needs a transform of 2^%d coefficients; the no-allocation "
+                    "entry point takes at most 2^%d (na + nb <= %zu limbs)",
+                    na, nb, log_len(na, nb), NTT_NOALLOC_MAX_LOG, (size_t)1 << (NTT_NOALLOC_MAX_LOG - 2));
+    const size_t nout = na + nb;
+    if (na == 0 || nb == 0 || use_schoolbook(na, nb, nout))
+        return enqueue_mul(d_c, d_a, na, d_b, nb, 0);       /* no workspace on these paths */
+    const int logL = log_len(na, nb);
+    size_t rw, sb;
+    class_needs(logL, 1, (uint32_t)nout, 0, rw, sb);
+    if (rw > NOALLOC_RES_WORDS || sb > NOALLOC_CARRY_BYTES)
+        return fail(NTT_ERR_TOO_LARGE, "product of %zu x %zu limbs exceeds the no-allocation scratch", na, nb);
+    Workspace ws;
+    ws.res = g.noalloc_scratch;
+    ws.res_words = NOALLOC_RES_WORDS;
+    ws.scratch = (uint8_t *)(g.noalloc_scratch + NOALLOC_RES_WORDS);
+    ws.scratch_bytes = NOALLOC_CARRY_BYTES;
+    MulDesc one{};
+    one.a = d_a;
+    one.b = d_b;
+    one.c = d_c;
+    one.na = (uint32_t)na;
+    one.nb = (uint32_t)nb;
+    one.n_out = (uint32_t)nout;
+    g.dispatches++;
+    g.n_marks = 0;
+    HIP_TRY(mark("start"));
+    g.no_alloc = 1;
+    const int rc = enqueue_ntt_class(ws, nullptr, one, 1, logL, (uint32_t)nout, 0, g.stream, 0, 0,
+                                     (uint32_t)std::max(na, nb));
+    g.no_alloc = 0;
+    /* ws does not own the scratch: nothing to release */
+    ws.res = nullptr;
+    ws.scratch = nullptr;
     g.timing_pending = g.timing;
     return rc;
 }
@@ -839,6 +901,13 @@ int ntt_mul_device(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t
 {
     std::lock_guard<std::recursive_mutex> lk(g_lock);
     int rc = enqueue_mul(d_c, d_a, na, d_b, nb, 0);
+    return rc ? rc : sync_check();
+}
+
+int ntt_mul_device_noalloc(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb)
+{
+    std::lock_guard<std::recursive_mutex> lk(g_lock);
+    int rc = enqueue_mul_noalloc(d_c, d_a, na, d_b, nb);
     return rc ? rc : sync_check();
 }
 

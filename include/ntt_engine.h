@@ -65,6 +65,20 @@ int ntt_mul_device_async(uint64_t *d_c, const uint64_t *d_a, size_t na, const ui
 int ntt_sqr_device_async(uint64_t *d_c, const uint64_t *d_a, size_t na);
 int ntt_engine_sync(void);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * or host memory (code under hipGraph stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc): ntt_mul_device with the same arguments,
+ * results and synchronous integrity check, whose workspace is a static
+ * per-device scratch array of 24.8 MB instead of engine allocations.
+ * It calls no hipMalloc, hipFree or allocating library routine; the engine
+ * must have been initialised (ntt_engine_init allocates its tables there).
+ * Limit: transforms of at most 2^19 coefficients (4 (na + nb) <= 2^19, e.g.
+ * two 2^22-bit operands); a larger product returns NTT_ERR_TOO_LARGE.  Every
+ * call on a GPU shares that GPU's array and leaves its contents behind: calls
+ * are serialised by the engine lock and stream, and anything else that
+ * enqueued work on the same array concurrently would race. */
+int ntt_mul_device_noalloc(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb);
+
 /* ---- batches ---------------------------------------------------------------
  * One product of a batch: c[0..n_out) = a*b + add (add may be NULL, nadd 0).
  * All pointers are device pointers; n_out >= na + nb and >= nadd, and the
