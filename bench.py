@@ -28,8 +28,9 @@ Other workloads (reported under "workloads"; skip with --headline-only):
           64 GB working-memory budget, swept arrival rates: products/s, p50/p99
   convert bigint_to_decimal of a 10^6-digit integer and newton_reciprocal of a
           2^22-bit integer (the reference's base_convert.h / newton.h entry points)
-CPU baseline: the oracle's 3-prime NTT multiply on a 2^26-bit sample scaled
-to 2^30 bits by L log L, with GMP mpz_mul on the same sample beside it.
+CPU baseline: the oracle's 3-prime NTT multiply of the headline's 2^30-bit
+operands on up to 16 host threads, measured at full size, with GMP mpz_mul of
+the same operands (one core) beside it.
 """
 import argparse
 import ctypes
@@ -63,6 +64,8 @@ def parse():
                     help="run warmup + steps of the product only (for rocprofv3), print nothing else")
     ap.add_argument("--profile-op", choices=("mul", "sqr"), default="mul",
                     help="with --profile-only: the product (mul) or the square of the first operand (sqr)")
+    ap.add_argument("--tree-compare", type=int, default=0, metavar="DIGITS",
+                    help="only compare the batched and per-product product trees of e at DIGITS digits")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one 1 GiB device-to-device copy (torch) to calibrate rocprofv3 byte counters")
     return ap.parse_args()
@@ -165,7 +168,12 @@ def headline(args, E, O):
     dc = E.DeviceBuffer(na + nb)
     for _ in range(max(1, args.warmup)):
         E.mul_device(dc, da, na, db, nb)
-    if not np.array_equal(dc.download(), O.gmp_mul(a, b)):
+    # GMP's product is the parity check; its time (one core, limb import and
+    # export included) is reported beside the CPU baseline
+    t0 = time.perf_counter()
+    want = O.gmp_mul(a, b)
+    t_gmp_mul = time.perf_counter() - t0
+    if not np.array_equal(dc.download(), want):
         raise SystemExit("bench.py: GPU product differs from GMP")
     if args.profile_only:
         for _ in range(args.steps):
@@ -193,8 +201,12 @@ def headline(args, E, O):
     L.ntt_engine_set_timing(0)
     for _ in range(2):
         E.sqr_device(dc, da, na)
-    if not np.array_equal(dc.download(), O.gmp_mul(a, a)):
+    t0 = time.perf_counter()
+    want_sq = O.gmp_mul(a, a)
+    t_gmp_sqr = time.perf_counter() - t0
+    if not np.array_equal(dc.download(), want_sq):
         raise SystemExit("bench.py: GPU square differs from GMP")
+    del want_sq
     t0 = time.perf_counter()
     for _ in range(args.steps):
         E.sqr_device(dc, da, na, sync=False)
@@ -230,7 +242,8 @@ def headline(args, E, O):
     roof["per_launch"] = [{"kernel": nm, "launch_ms": t, "alg_bytes": ab,
                            "achieved": ab / (t * 1e-3) / 1e9, "frac": ab / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
                           for (nm, _, ab), t in zip(table, times)]
-    return {"ms_per_step": ms_per_step, "square_ms": sq_ms, "host_api_ms": float(np.median(host_ms)), "logL": logL, "Llen": Llen, "nbits": nbits,
+    return {"operands": (a, b), "product": want, "gmp_mul_s": t_gmp_mul, "gmp_sqr_s": t_gmp_sqr,
+            "ms_per_step": ms_per_step, "square_ms": sq_ms, "host_api_ms": float(np.median(host_ms)), "logL": logL, "Llen": Llen, "nbits": nbits,
             "kernels_ms": kern, "total_event_ms": float(np.median(tot_ms)), "roofline": roof}
 
 
@@ -302,8 +315,12 @@ def workload_batch(E, O):
     t_capture = time.perf_counter() - t0
     g.launch()
     EN.sync()
-    bad = sum(1 for (a, b, oc) in host
-              if not np.array_equal(pool.download(oc, len(a) + len(b)), O.gmp_mul(a, b)))
+    got = [pool.download(oc, len(a) + len(b)) for (a, b, oc) in host]
+    t0 = time.perf_counter()
+    want = [O.gmp_mul(a, b) for (a, b, _) in host]      # GMP on one core: the CPU beside it
+    t_gmp = time.perf_counter() - t0
+    bad = sum(1 for g_, w_ in zip(got, want) if not np.array_equal(g_, w_))
+    del got, want
     reps = 5
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -323,7 +340,8 @@ def workload_batch(E, O):
     pool.close()
     return {"products": n, "all_bit_exact_vs_gmp": bad == 0, "mismatches": bad, "graph_ms": t * 1e3,
             "products_per_s": n / t, "ntt_alg_gbs": ntt_bytes / t / 1e9, "capture_ms": t_capture * 1e3,
-            "unbatched_ms": t_each * 1e3, "speedup_vs_unbatched": t_each / t, "schoolbook_threshold_limbs": thr}
+            "unbatched_ms": t_each * 1e3, "speedup_vs_unbatched": t_each / t, "schoolbook_threshold_limbs": thr,
+            "cpu_gmp_ms": t_gmp * 1e3, "cpu_gmp_note": "GMP mpz_mul of the same 4096 products, one core"}
 
 
 # ---- config 3: e ------------------------------------------------------------
@@ -342,16 +360,48 @@ def workload_e(E, O, digits):
     out["sha256_matches_gmp"] = bool(ent and hashlib.sha256(s.encode()).hexdigest() == ent["sha256"])
     comp = {}
     for d in (10 ** 6, 10 ** 7):
-        n = O.orc().orc_split_terms_needed(d + 20)
-        t_graph = EN.split_tree_time(1, n + 1, 1, runs=3)
-        t_each = EN.split_tree_time(1, n + 1, 2, runs=1)
-        comp[str(d)] = {"terms": n, "batched_graph_s": t_graph, "per_product_s": t_each,
-                        "speedup": t_each / t_graph}
+        comp[str(d)] = tree_compare(EN, O, d)
     out["tree_batched_vs_per_product"] = comp
     # CPU beside it: GMP binary splitting + division + conversion, one core, at 10^6 digits
     t0 = time.perf_counter()
     O.gmp_e_digits(10 ** 6)
     out["cpu_gmp_e_1e6_s"] = time.perf_counter() - t0
+    return out
+
+
+def tree_compare(EN, O, digits, per_product=True):
+    """The product tree of e at `digits` digits, level-batched in one hipGraph
+    against one launch sequence per product (the reference's plain path on the
+    same device arena): end to end (planning, and graph capture for the
+    batched tree, included) and evaluation alone, each on its own tree."""
+    import ctypes
+    L = EN.lib()
+    vp = ctypes.c_void_p
+    L.ntt_split_tree_create.argtypes = [ctypes.c_long, ctypes.c_long, ctypes.c_int, ctypes.POINTER(vp)]
+    L.ntt_split_tree_run.argtypes = [vp]
+    L.ntt_split_tree_destroy.argtypes = [vp]
+    n = O.orc().orc_split_terms_needed(digits + 20)
+
+    def leg(mode):
+        h = vp()
+        t0 = time.perf_counter()
+        EN.check(L.ntt_split_tree_create(1, n + 1, mode, ctypes.byref(h)))
+        t1 = time.perf_counter()
+        EN.check(L.ntt_split_tree_run(h))
+        t2 = time.perf_counter()
+        runs = 3 if mode == 1 else 1
+        for _ in range(runs if mode == 1 else 0):
+            EN.check(L.ntt_split_tree_run(h))
+        t3 = time.perf_counter()
+        L.ntt_split_tree_destroy(h)
+        return {"plan_s": t1 - t0, "first_run_s": t2 - t1, "end_to_end_s": t2 - t0,
+                "replay_s": (t3 - t2) / runs if mode == 1 else t2 - t1}
+
+    out = {"terms": n, "batched": leg(1)}
+    if per_product:
+        out["per_product"] = leg(2)
+        out["speedup_end_to_end"] = out["per_product"]["end_to_end_s"] / out["batched"]["end_to_end_s"]
+        out["speedup_evaluation"] = out["per_product"]["first_run_s"] / out["batched"]["replay_s"]
     return out
 
 
@@ -407,51 +457,100 @@ def workload_convert(E, O):
 # ---- config 5: serving -----------------------------------------------------
 
 def workload_serve(E, O):
+    """Poisson arrivals of requests with operands log-uniform in [2^10, 2^28] bits,
+    resident in HBM, through a server with a 64 GB budget and three slots, at
+    swept arrival rates; then the same requests all at once (the service's
+    capacity, which bounds what any arrival rate can be delivered).  The last
+    30 requests' products are checked against the same products alone (and
+    GMP where both operands are at most 2^24 bits)."""
     from cryptography_amd import engine as EN
+    import _workloads as W
     rng = np.random.default_rng(55)
     src_words = 1 << 24                    # operands: windows of 128 MiB of resident random limbs
     out_words = 1 << 28                    # 2 GiB of result space, reused round-robin
     pool = EN.DevicePool(src_words + out_words + 64)
     src = pool.take(src_words)
-    pool.upload(src, O.splitmix(77, src_words))
+    src_host = O.splitmix(77, src_words)
+    pool.upload(src, src_host)
     out_base = pool.take(out_words)
-    res = {}
-    for qps in (200, 1000, 5000):
-        n = int(min(1500, max(300, qps)))
-        bits = np.floor(2.0 ** (10 + 18 * rng.random((n, 2)))).astype(np.int64)
+
+    def requests(n, seed):
+        bits = W.config5_bits(n, seed)
         items, used = [], 0
         for i in range(n):
             la, lb = int((bits[i, 0] + 63) // 64), int((bits[i, 1] + 63) // 64)
             if used + la + lb > out_words:
                 used = 0
-            oa = src + int(rng.integers(0, src_words - la))
-            ob = src + int(rng.integers(0, src_words - lb))
-            items.append((pool.ptr(oa), la, pool.ptr(ob), lb, pool.ptr(out_base + used), la + lb, None, 0))
+            oa = int(rng.integers(0, src_words - la))
+            ob = int(rng.integers(0, src_words - lb))
+            items.append((pool.ptr(src + oa), la, pool.ptr(src + ob), lb, pool.ptr(out_base + used), la + lb,
+                          None, 0, oa, ob, used))
             used += la + lb
+        return items
+
+    res = {}
+    last = None
+    for qps in (100, 300, 1000, 3000):
+        n = int(min(1200, max(300, qps)))
+        items = requests(n, 1000 + qps)
         arrival = np.cumsum(rng.exponential(1.0 / qps, size=n))
-        t_done, st = EN.serve_replay(EN.make_descs(items), arrival, 64 << 30, slots=3)
+        t_done, st = EN.serve_replay(EN.make_descs([it[:8] for it in items]), arrival, 64 << 30, slots=3)
         lat = (t_done - arrival) * 1e3
-        res[str(qps)] = {"requests": n, "products_per_s": n / st["makespan_s"],
+        res[str(qps)] = {"requests": n, "offered_per_s": qps, "products_per_s": n / st["makespan_s"],
                          "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                          "batches": st["batches"], "max_inflight_batches": st["max_inflight"]}
+        last = items
+    # checks: the last requests of the last sweep point (before the capacity run
+    # below reuses their result space)
+    ok_alone, ok_gmp, n_gmp = True, True, 0
+    for (pa, la, pb, lb, pc, n_out, _, _, oa, ob, oc) in last[-30:]:
+        got = pool.download(out_base + oc, n_out)
+        a, b = src_host[oa:oa + la], src_host[ob:ob + lb]
+        with E.DeviceBuffer(n_out) as dc:
+            E.engine.check(E.lib().ntt_mul_device(dc.ptr, pa, la, pb, lb))
+            ok_alone &= bool(np.array_equal(got, dc.download()))
+        if max(la, lb) <= (1 << 18):
+            ok_gmp &= bool(np.array_equal(got, O.gmp_mul(a, b)))
+            n_gmp += 1
+    # capacity: every request queued at once
+    items = requests(1200, 7)
+    _, st = EN.serve_replay(EN.make_descs([it[:8] for it in items]), np.zeros(len(items)), 64 << 30, slots=3)
+    capacity = len(items) / st["makespan_s"]
     pool.close()
-    return {"budget_gb": 64, "slots": 3, "operand_bits": "2^(10 + 18u)", "by_qps": res}
+    return {"budget_gb": 64, "slots": 3, "operand_bits": "2^(10 + 18u)", "by_qps": res,
+            "capacity_products_per_s": capacity, "capacity_batches": st["batches"],
+            "checked_last": 30, "equal_alone": ok_alone, "equal_gmp": ok_gmp, "gmp_checked": n_gmp}
 
 
-def cpu_baseline(O, sample_bits=26, target_bits=30):
-    a = O.random_operand(101, 1 << sample_bits)
-    b = O.random_operand(102, 1 << sample_bits)
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(O, h):
+    """The oracle's 3-prime CT-DIT NTT multiply (the reference's algorithm, restated)
+    on the headline's own 2^30-bit operands, every transform spread over the host
+    cores this GPU's share of the box gives (at most 16), measured once at full
+    size; its product must equal GMP's.  GMP mpz_mul (one core) beside it."""
+    a, b = h["operands"]
+    threads = max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
-    O.mul_ntt3(a, b, threads=3)
+    c = O.mul_ntt3_par(a, b, threads)
     t_port = time.perf_counter() - t0
-    t_gmp = O.gmp().gmpref_time_mul(O._ptr(a), len(a), O._ptr(b), len(b), 1)
-    ls, lt = sample_bits - 3, target_bits - 3        # log2 transform lengths
-    scale = (2 ** lt * lt) / (2 ** ls * ls)
-    return {"value": t_port * scale * 1e3, "unit": "ms/mul (2^30-bit, scaled from sample)", "cores": 3,
-            "kind": "port",
-            "sample": f"one 2^{sample_bits}-bit x 2^{sample_bits}-bit product (L=2^{ls}) with the oracle's "
-                      f"3-prime NTT multiply, 3 OpenMP threads: {t_port * 1e3:.1f} ms; scaled by L log L x{scale:.2f}",
-            "gmp_sample_ms": t_gmp * 1e3, "nproc": os.cpu_count()}
+    exact = bool(np.array_equal(c, h["product"]))
+    return {"value": t_port * 1e3, "unit": "ms/mul (2^30-bit)", "cores": threads, "kind": "port",
+            "sample": f"one full 2^30-bit x 2^30-bit product (the headline operands, L = 2^27, 3 primes) with "
+                      f"the oracle's CT-DIT NTT multiply, every transform on {threads} OpenMP threads; "
+                      f"product equals GMP's: {exact}",
+            "exact": exact,
+            "gmp_mul_ms": h["gmp_mul_s"] * 1e3, "gmp_sqr_ms": h["gmp_sqr_s"] * 1e3,
+            "gmp_note": "GMP 6.2.1 mpz_mul on one core, the same operands (limb import and export included)",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
 
 def main():
@@ -471,6 +570,11 @@ def main():
     import cryptography_amd as E
     import _oracle as O
     E.init()
+    if args.tree_compare:
+        from cryptography_amd import engine as EN
+        progress(f"product tree of e at {args.tree_compare} digits: batched, then per product")
+        print(json.dumps({"tree_compare": tree_compare(EN, O, args.tree_compare), "digits": args.tree_compare}))
+        return
     progress("headline: 2^30-bit product")
     h = headline(args, E, O)
     if args.profile_only:
@@ -502,7 +606,7 @@ def main():
         line["workloads"] = wl
     if not args.no_cpu:
         progress("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(O)
+        line["cpu_baseline"] = cpu_baseline(O, h)
     print(json.dumps(line))
 
 

@@ -166,9 +166,20 @@ const uint64_t *xtab_for(int logS, int logT, hipStream_t st);
 /* Make the tables the passes of a length-2^logL transform use (before a
  * graph capture, which cannot allocate). */
 void xtab_prepare(int logL);
+/* b_res: b's rows after all its forward passes but the last (a resident
+ * spectrum, spectrum_rows_len words per prime), shared by every product of the
+ * class; only a is transformed and the workspace holds one operand's rows.
+ * fwd_only: run only those forward passes, of operand a (square = 1), into
+ * the workspace rows: that makes such a spectrum.  Both need
+ * spectrum_cacheable(logL). */
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
                       uint32_t max_nout, int square, hipStream_t st, int residues_in = 0, uint32_t b_period = 0,
-                      uint32_t max_nab = 0);
+                      uint32_t max_nab = 0, const uint64_t *b_res = nullptr, int fwd_only = 0);
+/* Whether a length-2^logL transform keeps an operand's partial spectrum in
+ * whole tiles between passes (two or more passes, L >= 4096). */
+bool spectrum_cacheable(int logL);
+/* Words per prime of such a spectrum (one product's padded row). */
+size_t spectrum_rows_len(int logL);
 int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp);
 
 /* Host threads for planning work (descriptor sorting, tree construction):
@@ -195,6 +206,8 @@ unsigned par_chunks(size_t n, F &&f, size_t min_chunk = (size_t)1 << 15)
     return nt;
 }
 int enqueue_batch(Workspace &ws, const BatchPlan &bp, const MulDesc *d_desc, hipStream_t st);
+/* Whether a product of na x nb limbs runs on the schoolbook kernel. */
+bool use_schoolbook(size_t na, size_t nb, size_t n_out);
 int sync_check();
 /* force_ntt: the CRT-NTT path whatever the operand sizes (no schoolbook). */
 int enqueue_mul(uint64_t *d_c, const uint64_t *d_a, size_t na, const uint64_t *d_b, size_t nb, int square,

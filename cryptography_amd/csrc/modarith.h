@@ -39,10 +39,39 @@ __host__ __device__ constexpr uint64_t neg_inv64(uint64_t p)
     return (uint64_t)0 - x;
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+/* gfx950 keeps 64-bit operands in even-aligned register pairs, so a 32-bit
+ * value used as a zero-extended 64-bit addend of v_mad_u64_u32 costs register
+ * moves when it sits in the odd half of a pair (the high word of a product).
+ * The two partial products of weight 2^32 of a 64 x 64-bit product are
+ * therefore summed with their full 64-bit partner in one v_mad_u64_u32 whose
+ * carry out (a lane mask) goes into the top word afterwards.
+ * mad_carry: y = a*b + x (mod 2^64) and its carry out. */
+__device__ __forceinline__ uint64_t mad_carry(uint32_t a, uint32_t b, uint64_t x, uint64_t &carry)
+{
+    uint64_t y;
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(y), "=s"(carry) : "v"(a), "v"(b), "v"(x));
+    return y;
+}
+/* z + carry * 2^32 (the carry out of mad_carry). */
+__device__ __forceinline__ uint64_t add_carry_hi(uint64_t z, uint64_t carry)
+{
+    uint32_t hi;
+    uint64_t c2;
+    asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(hi), "=s"(c2) : "v"((uint32_t)(z >> 32)), "s"(carry));
+    return ((uint64_t)hi << 32) | (uint32_t)z;
+}
+#endif
+
 __host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __umul64hi(a, b);
+    /* hi64(a*b) = a1*b1 + floor((a1*b0 + X) / 2^32), X = a0*b1 + hi32(a0*b0) < 2^64 */
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t X = (uint64_t)a0 * b1 + __umulhi(a0, b0);
+    uint64_t c;
+    const uint64_t Y = mad_carry(a1, b0, X, c);
+    return add_carry_hi((uint64_t)a1 * b1 + (Y >> 32), c);
 #else
     return (uint64_t)(((unsigned __int128)a * b) >> 64);
 #endif
@@ -172,8 +201,14 @@ __host__ __device__ __forceinline__ uint64_t mont_mul_lazy(uint64_t a, uint64_t 
     const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
     const uint64_t p00 = (uint64_t)a0 * b0;
     const uint64_t t = (uint64_t)a0 * b1 + (p00 >> 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint64_t cy;   /* u = a1*b0 + t as a 65-bit sum: low 64 bits and carry */
+    const uint64_t u = mad_carry(a1, b0, t, cy);
+    const uint64_t hi = add_carry_hi((uint64_t)a1 * b1 + (u >> 32), cy);
+#else
     const uint64_t u = (uint64_t)a1 * b0 + (uint32_t)t;
     const uint64_t hi = (uint64_t)a1 * b1 + (t >> 32) + (u >> 32);
+#endif
     const uint32_t l0 = (uint32_t)p00, l1 = (uint32_t)u;
     const uint64_t dd = (uint64_t)l0 * NPH + l1;                   /* D + l0*2^32 */
     const uint32_t s0 = (uint32_t)dd;                              /* D mod 2^32 */

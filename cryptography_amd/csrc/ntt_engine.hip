@@ -315,17 +315,23 @@ void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t
 
 /* Enqueue one size class: nprod products, transform length 2^logL, all on
  * `st`.  d_desc == nullptr means the single product `one`. */
+bool spectrum_cacheable(int logL) { return logL >= 12 && plan_passes(logL).npass >= 2; }
+size_t spectrum_rows_len(int logL) { return row_len(logL, 1); }
+
 int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, uint32_t nprod, int logL,
                       uint32_t max_nout, int square, hipStream_t st, int residues_in, uint32_t b_period,
-                      uint32_t max_nab)
+                      uint32_t max_nab, const uint64_t *b_res, int fwd_only)
 {
     const size_t L = (size_t)1 << logL;
     const size_t len = row_len(logL, nprod);
     const int nops = square ? 1 : 2;
     const uint32_t pad_log = pad_log_for(logL);
+    if ((b_res || fwd_only) && (!spectrum_cacheable(logL) || residues_in || (b_res && (square || fwd_only)) ||
+                                (fwd_only && (!square || nprod != 1))))
+        return fail(NTT_ERR_ARG, "resident spectrum not usable for a class of length 2^%d (internal)", logL);
     size_t need_res, need_scr;
-    class_needs(logL, nprod, max_nout, square, need_res, need_scr);
-    if (ws.res_words < need_res || ws.scratch_bytes < need_scr)
+    class_needs(logL, nprod, max_nout, square || b_res, need_res, need_scr);
+    if (ws.res_words < need_res || (!fwd_only && ws.scratch_bytes < need_scr))
         return fail(NTT_ERR_ARG, "workspace too small for the class (internal)");
     const PassPlan pp = plan_passes(logL);
     const size_t ntiles = row_residues(logL, nprod) / FT_TILE;
@@ -374,9 +380,9 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
      * transformed for the first P products only; the fused pass reads tile
      * t mod (P L / 4096) of those rows.  Needs whole tiles per product
      * (L >= 4096) and separate passes. */
-    const bool share = b_period && b_period < nprod && nprod % b_period == 0 && !square && !residues_in &&
-                       pp.npass >= 2 && logL >= 12;
-    const int fwd_ops = share ? 1 : nops;
+    const bool share = !b_res && b_period && b_period < nprod && nprod % b_period == 0 && !square &&
+                       !residues_in && pp.npass >= 2 && logL >= 12;
+    const int fwd_ops = (share || b_res) ? 1 : nops;
     uint32_t done = 0;
     for (int i = 0; i < pp.npass - 1; i++) {
         geom(i, done);
@@ -401,6 +407,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         }
         done += pp.logS[i];
     }
+    if (fwd_only) return NTT_OK;         /* the rows hold a's partial spectrum */
     /* the last inverse pass (pass 0's) applies the 1/L scale when it loads
      * its tile with the cross twiddles (kinds 5 and 7: its chain's first
      * twiddle or its table carries the scale); otherwise the middle pass does */
@@ -411,10 +418,13 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     pa.half_zero = 0;
     pa.tpb = tiles_per_block(ntiles, 0);
     pa.first = pp.npass == 1 && !residues_in;
-    pa.b_tiles = share ? (uint32_t)(b_period * (L / FT_TILE)) : 0;
+    pa.b_tiles = share ? (uint32_t)(b_period * (L / FT_TILE)) : b_res ? (uint32_t)(L / FT_TILE) : 0;
+    pa.bres = b_res;
+    pa.blen = b_res ? spectrum_rows_len(logL) : 0;
     pa.scale_late = scale_late;
     HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], pa.first ? 4 : 2, pa, ntiles, nops, st));
     pa.b_tiles = 0;
+    pa.bres = nullptr;
     pa.scale_late = 0;
     HIP_TRY(mark("k_fast_pass<mid>"));
     for (int i = pp.npass - 2; i >= 0; i--) {

@@ -1007,7 +1007,7 @@ k_fast_pass(PassArgs pa)
     static_assert((1 << LOGS) <= TW_LDS_MAX, "stage twiddles exceed LDS table");
     const uint32_t pi = blockIdx.y, op = blockIdx.z + pa.op0;
     uint64_t *xa = pa.res + ((size_t)op * NTT_NPRIMES + pi) * pa.len;
-    const uint64_t *xb = pa.res + ((size_t)NTT_NPRIMES + pi) * pa.len;
+    const uint64_t *xb = pa.bres ? pa.bres + (size_t)pi * pa.blen : pa.res + ((size_t)NTT_NPRIMES + pi) * pa.len;
     if (pi == 0) pass_body<0, LOGS, KIND>(pa, xa, xb, op, lds, tws);
     else if (pi == 1) pass_body<1, LOGS, KIND>(pa, xa, xb, op, lds, tws);
     else pass_body<2, LOGS, KIND>(pa, xa, xb, op, lds, tws);

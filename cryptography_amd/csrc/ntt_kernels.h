@@ -68,6 +68,12 @@ struct PassArgs {
                                    kind 7 reads a table that includes it */
     const uint64_t *xtab;       /* kinds 6 and 7: this pass's cross twiddles, one
                                    entry per residue of an S x T block, per prime */
+    const uint64_t *bres;       /* fused pass: b's rows (after all forward passes
+                                   but the last) kept outside the class, one
+                                   product long, prime i at bres + i*blen; every
+                                   product reads them (b_tiles = L/4096); nullptr:
+                                   b is the class's second operand */
+    size_t blen;
 };
 
 #define FT_TILE 4096            /* residues per transform tile */

@@ -359,22 +359,102 @@ def get_tile_batch():
 
 
 class ServeStats(ctypes.Structure):
-    _fields_ = [("makespan_s", ctypes.c_double), ("batches", ctypes.c_size_t), ("max_inflight", ctypes.c_size_t)]
+    _fields_ = [("makespan_s", ctypes.c_double), ("batches", ctypes.c_size_t), ("max_inflight", ctypes.c_size_t),
+                ("completed", ctypes.c_size_t), ("spectrum_hits", ctypes.c_size_t),
+                ("spectra_made", ctypes.c_size_t)]
+
+
+def _stats_dict(st):
+    return {k: getattr(st, k) for k, _ in ServeStats._fields_}
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_inflight", ctypes.c_int), ("max_batch", ctypes.c_uint32)]
+
+
+def _serve_lib():
+    L = lib()
+    vp, sz, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64
+    L.ntt_serve_open.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(vp)]
+    L.ntt_serve_submit.argtypes = [vp, ctypes.POINTER(MulDesc), ctypes.POINTER(u64)]
+    L.ntt_serve_poll.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    L.ntt_serve_wait.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    L.ntt_serve_pin.argtypes = [vp, vp, ctypes.c_uint32]
+    L.ntt_serve_unpin.argtypes = [vp, vp, ctypes.c_uint32]
+    L.ntt_serve_get_stats.argtypes = [vp, ctypes.POINTER(ServeStats)]
+    L.ntt_serve_close.argtypes = [vp]
+    L.ntt_serve_replay.argtypes = [ctypes.POINTER(MulDesc), ctypes.POINTER(ctypes.c_double), sz, sz, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ServeStats)]
+    return L
+
+
+class Server:
+    """A continuous-batching server (include/ntt_serve.h): submit() from any
+    thread, then poll() or wait() on the ticket."""
+
+    def __init__(self, hbm_budget, max_inflight=3, max_batch=0):
+        init()
+        self.L = _serve_lib()
+        cfg = ServeConfig(hbm_budget, max_inflight, max_batch)
+        h = ctypes.c_void_p()
+        check(self.L.ntt_serve_open(ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+
+    def submit(self, desc):
+        """desc: a MulDesc; returns its ticket."""
+        t = ctypes.c_uint64()
+        check(self.L.ntt_serve_submit(self.h, ctypes.byref(desc), ctypes.byref(t)))
+        return t.value
+
+    def _done(self, f, ticket):
+        st, td = ctypes.c_int(), ctypes.c_double()
+        r = f(self.h, ticket, ctypes.byref(st), ctypes.byref(td))
+        if r < 0:
+            check(-r)
+        return (r == 1), st.value, td.value
+
+    def poll(self, ticket):
+        """(done, status, t_done seconds since open); a done ticket is retired."""
+        return self._done(self.L.ntt_serve_poll, ticket)
+
+    def wait(self, ticket):
+        """(status, t_done) once the request is complete."""
+        _, st, td = self._done(self.L.ntt_serve_wait, ticket)
+        return st, td
+
+    def pin(self, ptr, n):
+        check(self.L.ntt_serve_pin(self.h, ptr, n))
+
+    def unpin(self, ptr, n):
+        check(self.L.ntt_serve_unpin(self.h, ptr, n))
+
+    def stats(self):
+        st = ServeStats()
+        check(self.L.ntt_serve_get_stats(self.h, ctypes.byref(st)))
+        return _stats_dict(st)
+
+    def close(self):
+        if self.h:
+            self.L.ntt_serve_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def serve_replay(descs, arrival, hbm_budget, slots=3):
-    """Replay an arrival stream of batch descriptors; returns (t_done, stats)."""
-    L = lib()
+    """Replay an arrival stream of batch descriptors through a server; returns (t_done, stats)."""
+    L = _serve_lib()
     init()
-    L.ntt_serve_replay.argtypes = [ctypes.POINTER(MulDesc), ctypes.POINTER(ctypes.c_double), ctypes.c_size_t,
-                                   ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_double),
-                                   ctypes.POINTER(ServeStats)]
     arr = np.ascontiguousarray(arrival, dtype=np.float64)
     out = np.zeros(len(descs), dtype=np.float64)
     st = ServeStats()
     check(L.ntt_serve_replay(descs, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(descs), hbm_budget,
                              slots, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(st)))
-    return out, {"makespan_s": st.makespan_s, "batches": st.batches, "max_inflight": st.max_inflight}
+    return out, _stats_dict(st)
 
 
 def split_tree_time(a, b, mode, runs=1):

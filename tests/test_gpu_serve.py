@@ -1,22 +1,30 @@
-"""GPU tests of the continuous-batching service: every result equals the same
-product computed alone (and GMP), whatever mix it was served in, including
-under an HBM budget small enough to make requests wait for admission."""
+"""GPU tests of the continuous-batching service (include/ntt_serve.h): every
+result equals the same product computed alone (and GMP), whatever mix it was
+served in, under budgets that make requests wait for admission, with
+submitters on several threads, and with a pinned operand whose resident
+spectrum replaces its transform."""
+import threading
+
 import numpy as np
 import pytest
 
 import _oracle as O
+import _workloads as W
 import cryptography_amd as E
 from cryptography_amd import engine as EN
 
 pytestmark = pytest.mark.gpu
 
 
-def make_requests(pool, n, seed, lo_bits=10, hi_bits=20):
+def make_requests(pool, n, seed, lo_bits=10, hi_bits=20, bits=None):
     rng = np.random.default_rng(seed)
     items, host = [], []
     for i in range(n):
-        ba = int(2 ** (lo_bits + (hi_bits - lo_bits) * rng.random()))
-        bb = int(2 ** (lo_bits + (hi_bits - lo_bits) * rng.random()))
+        if bits is None:
+            ba = int(2 ** (lo_bits + (hi_bits - lo_bits) * rng.random()))
+            bb = int(2 ** (lo_bits + (hi_bits - lo_bits) * rng.random()))
+        else:
+            ba, bb = int(bits[i, 0]), int(bits[i, 1])
         a = O.random_operand(seed * 100000 + 2 * i, ba)
         b = O.random_operand(seed * 100000 + 2 * i + 1, bb)
         oa, ob, oc = pool.take(len(a)), pool.take(len(b)), pool.take(len(a) + len(b))
@@ -27,6 +35,14 @@ def make_requests(pool, n, seed, lo_bits=10, hi_bits=20):
     return EN.make_descs(items), host
 
 
+def alone(pool, a, b):
+    """The same product through the plain device multiply (ntt_mul_device)."""
+    with E.DeviceBuffer.from_array(a) as da, E.DeviceBuffer.from_array(b) as db, \
+            E.DeviceBuffer(len(a) + len(b)) as dc:
+        E.mul_device(dc, da, len(a), db, len(b))
+        return dc.download()
+
+
 @pytest.mark.parametrize("budget_mb,slots", [(4096, 3), (48, 2)])
 def test_served_results_equal_alone(budget_mb, slots):
     pool = EN.DevicePool(40_000_000)
@@ -35,12 +51,135 @@ def test_served_results_equal_alone(budget_mb, slots):
         rng = np.random.default_rng(7)
         arrival = np.cumsum(rng.exponential(1.0 / 20000.0, size=len(descs)))
         t_done, st = EN.serve_replay(descs, arrival, budget_mb << 20, slots=slots)
-        assert st["batches"] >= 1
+        assert st["batches"] >= 1 and st["completed"] == len(descs)
         assert np.all(t_done >= arrival - 1e-6)
         for (a, b, oc) in host:
             got = pool.download(oc, len(a) + len(b))
             assert np.array_equal(got, O.gmp_mul(a, b))
             assert np.array_equal(got, E.mul(a, b))        # the same product alone
+    finally:
+        pool.close()
+
+
+@pytest.mark.timeout(600)
+def test_config5_requests_up_to_2p28_bits_under_64gb():
+    """The serving configuration's sizes: operands log-uniform in [2^10, 2^28]
+    bits, Poisson arrivals, a 64 GB budget, three slots.  Each result equals
+    GMP's product and the same product computed alone."""
+    n = 24
+    bits = W.config5_bits(n, seed=5)
+    words = int(((bits + 63) // 64).sum() * 2) + 64
+    pool = EN.DevicePool(words)
+    try:
+        descs, host = make_requests(pool, n, seed=55, bits=bits)
+        arrival = np.cumsum(np.random.default_rng(8).exponential(1.0 / 400.0, size=n))
+        t_done, st = EN.serve_replay(descs, arrival, 64 << 30, slots=3)
+        assert st["completed"] == n and np.all(t_done >= arrival - 1e-6)
+        print("batches", st["batches"], "max in flight", st["max_inflight"], flush=True)
+        for i, (a, b, oc) in enumerate(host):
+            got = pool.download(oc, len(a) + len(b))
+            assert np.array_equal(got, O.gmp_mul(a, b)), i
+            assert np.array_equal(got, alone(pool, a, b)), i
+    finally:
+        pool.close()
+
+
+def test_submitters_on_several_threads():
+    """Four threads submit 40 requests each while earlier ones run; every
+    ticket completes with NTT_OK and every product equals GMP's."""
+    pool = EN.DevicePool(30_000_000)
+    try:
+        descs, host = make_requests(pool, 160, seed=21, lo_bits=10, hi_bits=21)
+        with EN.Server(1 << 30, max_inflight=4) as srv:
+            tickets = [None] * len(descs)
+            errors = []
+
+            def worker(k):
+                try:
+                    for i in range(k, len(descs), 4):
+                        tickets[i] = srv.submit(descs[i])
+                except Exception as e:        # surfaced in the main thread
+                    errors.append(e)
+
+            th = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            assert not errors
+            assert len(set(tickets)) == len(descs)
+            for t in tickets:
+                status, t_done = srv.wait(t)
+                assert status == 0 and t_done > 0
+            st = srv.stats()
+            assert st["completed"] == len(descs) and st["batches"] >= 1
+        for (a, b, oc) in host:
+            assert np.array_equal(pool.download(oc, len(a) + len(b)), O.gmp_mul(a, b))
+    finally:
+        pool.close()
+
+
+def test_pinned_operand_reuses_its_spectrum():
+    """A pinned 2^20-bit operand multiplied by 30 operands of three transform
+    lengths, on either side of the product and between unpinned requests:
+    its forward transform is made once per length and reused, and every
+    product equals GMP's and the unpinned path's."""
+    pool = EN.DevicePool(20_000_000)
+    try:
+        p = O.random_operand(99, 1 << 20)
+        op = pool.take(len(p))
+        pool.upload(op, p)
+        rng = np.random.default_rng(3)
+        items, host = [], []
+        for i in range(40):
+            pinned = i % 4 != 3
+            bits = int(2 ** (14 + 8 * rng.random()))
+            a = O.random_operand(500 + i, bits)
+            oa = pool.take(len(a))
+            pool.upload(oa, a)
+            if pinned:
+                b, ob = p, op
+            else:
+                b = O.random_operand(900 + i, bits // 3 + 64)
+                ob = pool.take(len(b))
+                pool.upload(ob, b)
+            oc = pool.take(len(a) + len(b))
+            if pinned and i % 2:     # the pinned operand as a
+                items.append((pool.ptr(ob), len(b), pool.ptr(oa), len(a), pool.ptr(oc), len(a) + len(b), None, 0))
+            else:
+                items.append((pool.ptr(oa), len(a), pool.ptr(ob), len(b), pool.ptr(oc), len(a) + len(b), None, 0))
+            host.append((a, b, oc, pinned))
+        descs = EN.make_descs(items)
+        n_pinned = sum(1 for h in host if h[3])
+        lens = {E.log_len(len(a), len(b)) for (a, b, _, pinned) in host if pinned}
+        with EN.Server(2 << 30, max_inflight=3) as srv:
+            srv.pin(pool.ptr(op), len(p))
+            tickets = [srv.submit(d) for d in descs]
+            for t in tickets:
+                assert srv.wait(t)[0] == 0
+            st = srv.stats()
+            assert st["spectrum_hits"] == n_pinned
+            assert st["spectra_made"] == len(lens)
+            srv.unpin(pool.ptr(op), len(p))
+            with pytest.raises(E.NttError):
+                srv.unpin(pool.ptr(op), len(p))       # no longer pinned
+        for (a, b, oc, pinned) in host:
+            got = pool.download(oc, len(a) + len(b))
+            assert np.array_equal(got, O.gmp_mul(a, b))
+    finally:
+        pool.close()
+
+
+def test_close_finishes_queued_requests():
+    pool = EN.DevicePool(10_000_000)
+    try:
+        descs, host = make_requests(pool, 60, seed=31, lo_bits=12, hi_bits=20)
+        srv = EN.Server(1 << 30, max_inflight=2)
+        for d in descs:
+            srv.submit(d)
+        srv.close()                  # returns once every queued request is done
+        for (a, b, oc) in host:
+            assert np.array_equal(pool.download(oc, len(a) + len(b)), O.gmp_mul(a, b))
     finally:
         pool.close()
 
@@ -52,6 +191,12 @@ def test_request_larger_than_budget_is_rejected():
         with pytest.raises(E.NttError) as ei:
             EN.serve_replay(descs, np.zeros(1), 1 << 20, slots=1)
         assert ei.value.code == 2
+        with EN.Server(1 << 20, max_inflight=1) as srv:
+            with pytest.raises(E.NttError) as ei:
+                srv.submit(descs[0])
+            assert ei.value.code == 2
+            with pytest.raises(E.NttError):
+                srv.poll(12345)                       # unknown ticket
     finally:
         pool.close()
 
