@@ -64,6 +64,8 @@ def parse():
                     help="run warmup + steps of the product only (for rocprofv3), print nothing else")
     ap.add_argument("--profile-op", choices=("mul", "sqr"), default="mul",
                     help="with --profile-only: the product (mul) or the square of the first operand (sqr)")
+    ap.add_argument("--tuning", default=os.path.join(ROOT, "cryptography_amd", "tuning_mi355x.json"),
+                    help="tuning file to load (ntt_engine_autotune_save format); 'none' keeps the defaults")
     ap.add_argument("--tree-compare", type=int, default=0, metavar="DIGITS",
                     help="only compare the batched and per-product product trees of e at DIGITS digits")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -337,10 +339,25 @@ def workload_batch(E, O):
     thr = L.ntt_engine_get_threshold()
     ntt_bytes = sum(b_alg(E.log_len(int(x), int(y))) for x, y in zip(la, lb) if max(x, y) > thr)
     g.close()
+    # the same graph at the reference's fixed crossover of 64 limbs (multiply.h:23-25)
+    t_default = None
+    if thr != 64:
+        L.ntt_engine_set_threshold(64)
+        g2 = EN.BatchGraph(descs)
+        g2.launch()
+        EN.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            g2.launch()
+        EN.sync()
+        t_default = (time.perf_counter() - t0) / reps
+        g2.close()
+        L.ntt_engine_set_threshold(thr)
     pool.close()
     return {"products": n, "all_bit_exact_vs_gmp": bad == 0, "mismatches": bad, "graph_ms": t * 1e3,
             "products_per_s": n / t, "ntt_alg_gbs": ntt_bytes / t / 1e9, "capture_ms": t_capture * 1e3,
             "unbatched_ms": t_each * 1e3, "speedup_vs_unbatched": t_each / t, "schoolbook_threshold_limbs": thr,
+            "graph_ms_at_threshold_64": None if t_default is None else t_default * 1e3,
             "cpu_gmp_ms": t_gmp * 1e3, "cpu_gmp_note": "GMP mpz_mul of the same 4096 products, one core"}
 
 
@@ -570,6 +587,11 @@ def main():
     import cryptography_amd as E
     import _oracle as O
     E.init()
+    tuning = None
+    if args.tuning != "none" and os.path.exists(args.tuning):
+        E.lib().ntt_engine_autotune_load.argtypes = [ctypes.c_char_p]
+        E.engine.check(E.lib().ntt_engine_autotune_load(args.tuning.encode()))
+        tuning = os.path.relpath(args.tuning, ROOT)
     if args.tree_compare:
         from cryptography_amd import engine as EN
         progress(f"product tree of e at {args.tree_compare} digits: batched, then per product")
@@ -593,6 +615,9 @@ def main():
         "kernels_ms": h["kernels_ms"],
         "total_event_ms": h["total_event_ms"],
         "roofline": h["roofline"],
+        "tuning": {"file": tuning, "threshold_limbs": E.lib().ntt_engine_get_threshold(),
+                   "tile_batch": E.lib().ntt_engine_get_tile_batch(),
+                   "pass_split_2p27": pass_plan(E.lib(), 27)},
     }
     if not args.headline_only:
         wl = {}

@@ -63,6 +63,87 @@ int time_batch(uint64_t *buf, size_t n, int count, int reps, double *sec)
     return NTT_OK;
 }
 
+/* Crossover measurements of the last autotune run, per operand size n:
+ * seconds per product of a batch of 256 n x n products on the schoolbook
+ * kernel and on the NTT, and a lower bound for one Karatsuba level over the
+ * faster of the two: a batch of 3 x 256 products of ceil(n/2)+1 limbs (the
+ * three half products a0 b0, a1 b1, (a0 + a1)(b0 + b1) with its carry limb),
+ * per product of size n, leaving out the additions and the recombination. */
+struct CrossSample {
+    int n;
+    double t_sb, t_ntt, t_kara_lb;
+};
+std::vector<CrossSample> g_cross;
+
+/* ---- a small JSON reader for the tuning file --------------------------------
+ * Values: numbers, strings, objects and arrays; the keys the engine reads
+ * are looked up in the top-level object, everything else is skipped. */
+struct JsonIn {
+    const char *p;
+    bool ok = true;
+    void ws()
+    {
+        while (*p == ' ' || *p == '\n' || *p == '\t' || *p == '\r') p++;
+    }
+    bool eat(char c)
+    {
+        ws();
+        if (*p != c) return false;
+        p++;
+        return true;
+    }
+    bool str(char *out, size_t cap)
+    {
+        ws();
+        if (*p != '"') return ok = false;
+        p++;
+        size_t n = 0;
+        while (*p && *p != '"') {
+            if (n + 1 < cap) out[n++] = *p;
+            p++;
+        }
+        if (*p != '"') return ok = false;
+        p++;
+        if (cap) out[n] = 0;
+        return true;
+    }
+    bool num(double &v)
+    {
+        ws();
+        char *end = nullptr;
+        v = strtod(p, &end);
+        if (end == p) return ok = false;
+        p = end;
+        return true;
+    }
+    void skip()   /* any value */
+    {
+        ws();
+        if (*p == '{') {
+            p++;
+            if (eat('}')) return;
+            do {
+                char k[64];
+                if (!str(k, sizeof k) || !eat(':')) { ok = false; return; }
+                skip();
+            } while (ok && eat(','));
+            if (!eat('}')) ok = false;
+        } else if (*p == '[') {
+            p++;
+            if (eat(']')) return;
+            do skip();
+            while (ok && eat(','));
+            if (!eat(']')) ok = false;
+        } else if (*p == '"') {
+            char k[256];
+            str(k, sizeof k);
+        } else {
+            double v;
+            num(v);
+        }
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -78,45 +159,71 @@ int ntt_engine_autotune_save(const char *path)
             fprintf(f, "%s\"%d\": %d", first ? "" : ", ", l, g.max_logs[l]);
             first = 0;
         }
-    fprintf(f, "}}\n");
+    fprintf(f, "}");
+    if (!g_cross.empty()) {
+        /* microseconds per product, for the record (the loader skips it) */
+        fprintf(f, ", \"crossover_us\": [");
+        for (size_t i = 0; i < g_cross.size(); i++)
+            fprintf(f, "%s[%d, %.4f, %.4f, %.4f]", i ? ", " : "", g_cross[i].n, g_cross[i].t_sb * 1e6,
+                    g_cross[i].t_ntt * 1e6, g_cross[i].t_kara_lb * 1e6);
+        fprintf(f, "]");
+    }
+    fprintf(f, "}\n");
     fclose(f);
     return NTT_OK;
 }
 
-/* Minimal reader for the file written above. */
+/* Reader for the file written above: {"threshold": T, "tile_batch": B,
+ * "pass_cap": {"<log_len>": cap, ...}, ...}; other keys are skipped, a
+ * missing tile_batch or pass_cap leaves the current value, and anything
+ * malformed or out of range is an error that changes nothing. */
 int ntt_engine_autotune_load(const char *path)
 {
     std::lock_guard<std::recursive_mutex> lk(g_lock);
     FILE *f = fopen(path, "r");
     if (!f) return fail(NTT_ERR_ARG, "cannot read %s", path);
-    char buf[4096];
-    size_t n = fread(buf, 1, sizeof buf - 1, f);
+    std::vector<char> buf(1 << 16);
+    size_t n = fread(buf.data(), 1, buf.size() - 1, f);
     fclose(f);
     buf[n] = 0;
-    const char *p = strstr(buf, "\"threshold\":");
-    if (!p) return fail(NTT_ERR_ARG, "%s: no threshold", path);
-    int thr = atoi(p + 12);
-    int tb = g.tile_batch;
-    const char *pt = strstr(buf, "\"tile_batch\":");
-    if (pt) tb = atoi(pt + 13);
-    if (tb < 1 || tb > 64) return fail(NTT_ERR_ARG, "%s: tile_batch %d out of range", path, tb);
-    uint8_t caps[NTT_MAX_LOG_LEN + 1] = {};
-    p = strstr(buf, "\"pass_cap\":");
-    if (p) {
-        p = strchr(p, '{');
-        while (p && *p && *p != '}') {
-            const char *q = strchr(p, '"');
-            if (!q) break;
-            int l = atoi(q + 1);
-            const char *c = strchr(q + 1, ':');
-            if (!c) break;
-            int cap = atoi(c + 1);
-            if (l >= 1 && l <= NTT_MAX_LOG_LEN && cap >= 4 && cap <= FT_MAX_LOGS) caps[l] = (uint8_t)cap;
-            p = strchr(c, ',');
-            if (!p) break;
-        }
+    JsonIn in{buf.data()};
+    int thr = -1, tb = g.tile_batch;
+    uint8_t caps[NTT_MAX_LOG_LEN + 1];
+    memcpy(caps, g.max_logs, sizeof caps);
+    if (!in.eat('{')) return fail(NTT_ERR_ARG, "%s: not a JSON object", path);
+    if (!in.eat('}')) {
+        do {
+            char key[64];
+            if (!in.str(key, sizeof key) || !in.eat(':')) break;
+            double v;
+            if (!strcmp(key, "threshold")) {
+                if (in.num(v)) thr = (int)v;
+            } else if (!strcmp(key, "tile_batch")) {
+                if (in.num(v)) tb = (int)v;
+            } else if (!strcmp(key, "pass_cap")) {
+                memset(caps, 0, sizeof caps);
+                if (!in.eat('{')) { in.ok = false; break; }
+                if (!in.eat('}')) {
+                    do {
+                        char lk_[16];
+                        if (!in.str(lk_, sizeof lk_) || !in.eat(':') || !in.num(v)) { in.ok = false; break; }
+                        const int l = atoi(lk_), cap = (int)v;
+                        if (l < 1 || l > NTT_MAX_LOG_LEN || cap < 4 || cap > FT_MAX_LOGS)
+                            return fail(NTT_ERR_ARG, "%s: pass cap %d for 2^%d out of range", path, cap, l);
+                        caps[l] = (uint8_t)cap;
+                    } while (in.ok && in.eat(','));
+                    if (in.ok && !in.eat('}')) in.ok = false;
+                }
+            } else {
+                in.skip();
+            }
+        } while (in.ok && in.eat(','));
+        if (in.ok && !in.eat('}')) in.ok = false;
     }
-    if (thr < 0 || thr > SB_MAX_OUT / 2) return fail(NTT_ERR_ARG, "%s: threshold %d out of range", path, thr);
+    if (!in.ok) return fail(NTT_ERR_ARG, "%s: malformed tuning file", path);
+    if (thr < 0) return fail(NTT_ERR_ARG, "%s: no threshold", path);
+    if (thr > SB_MAX_OUT / 2) return fail(NTT_ERR_ARG, "%s: threshold %d out of range", path, thr);
+    if (tb < 1 || tb > 64) return fail(NTT_ERR_ARG, "%s: tile_batch %d out of range", path, tb);
     g.threshold = thr;
     g.tile_batch = tb;
     memcpy(g.max_logs, caps, sizeof caps);
@@ -133,7 +240,7 @@ int ntt_engine_autotune(int min_log, int max_log)
     if (min_log < 11) min_log = 11;
     if (max_log > 28) max_log = 28;
     /* operand buffer: big enough for the largest single product and the batches */
-    size_t words = std::max((size_t)1 << (max_log - 1), (size_t)256 * 4 * 128);
+    size_t words = std::max((size_t)1 << (max_log - 1), (size_t)768 * 4 * 128);
     uint64_t *buf = nullptr;
     if (hipMalloc((void **)&buf, words * 8) != hipSuccess) return fail(NTT_ERR_NOMEM, "autotune buffer");
     hipLaunchKernelGGL(k_fill_pattern, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, g.stream, buf, words,
@@ -142,15 +249,21 @@ int ntt_engine_autotune(int min_log, int max_log)
     /* 1. schoolbook / NTT crossover */
     const int saved = g.threshold;
     int best_thr = 0;
+    g_cross.clear();
     for (int n : {8, 16, 24, 32, 48, 64, 96, 128}) {
-        double t_sb = 0, t_ntt = 0;
+        double t_sb = 0, t_ntt = 0, t_h_sb = 0, t_h_ntt = 0;
+        const int h = (n + 1) / 2 + 1;
         g.threshold = n;
         rc = time_batch(buf, n, 256, 3, &t_sb);
+        if (!rc) rc = time_batch(buf, h, 768, 3, &t_h_sb);
         if (rc) break;
         g.threshold = 0;
         rc = time_batch(buf, n, 256, 3, &t_ntt);
+        if (!rc) rc = time_batch(buf, h, 768, 3, &t_h_ntt);
         if (rc) break;
         if (t_sb <= t_ntt) best_thr = n;
+        /* per product of size n: three half products each */
+        g_cross.push_back({n, t_sb, t_ntt, 3 * std::min(t_h_sb, t_h_ntt)});
     }
     g.threshold = rc ? saved : best_thr;
     /* 2. pass caps: one product of L/8 x L/8 limbs per length */

@@ -5,8 +5,12 @@
  * (ntt/app/compute_e/main.c:155-165 arguments, :167-173 header line,
  * :340-345 the "e = 2.ddd..." line with D significant digits):
  *
- *   compute_e -d <digits> [-f|--force] [--plain]
+ *   compute_e -d <digits> [-f|--force] [--plain] [-t tuning.json]
  *
+ * -t loads a tuning file (ntt_engine_autotune_save's format) after init; by
+ * default the file tuning_mi355x.json beside the executable is loaded when it
+ * exists (the NTT_AMD_TUNING environment variable, read at init, takes
+ * precedence over both).
  * --plain evaluates the product tree with the reference's depth-first
  * recursion (one multiply call per product) instead of the level-batched
  * device tree.  -f is accepted for compatibility; there is no GPU-time gate.
@@ -23,6 +27,7 @@ int main(int argc, char **argv)
 {
     long n_digits = 1000;
     int batched = 1;
+    const char *tuning = NULL;
     for (int i = 1; i < argc; i++) {
         if (strcmp(argv[i], "-d") == 0 && i + 1 < argc) {
             n_digits = atol(argv[++i]);
@@ -34,8 +39,10 @@ int main(int argc, char **argv)
             /* accepted for compatibility */
         } else if (strcmp(argv[i], "--plain") == 0) {
             batched = 0;
+        } else if (strcmp(argv[i], "-t") == 0 && i + 1 < argc) {
+            tuning = argv[++i];
         } else {
-            fprintf(stderr, "Usage: %s [-d digits] [-f/--force] [--plain]\n", argv[0]);
+            fprintf(stderr, "Usage: %s [-d digits] [-f/--force] [--plain] [-t tuning.json]\n", argv[0]);
             return 1;
         }
     }
@@ -45,6 +52,22 @@ int main(int argc, char **argv)
     fflush(stdout);
     int rc = ntt_engine_init(0);
     if (rc) {
+        fprintf(stderr, "compute_e: %s\n", ntt_engine_last_error());
+        return 1;
+    }
+    /* the tuning file: -t, else tuning_mi355x.json beside the executable */
+    char beside[4096];
+    if (!tuning && !getenv("NTT_AMD_TUNING")) {
+        const char *slash = strrchr(argv[0], '/');
+        const int dir = slash ? (int)(slash - argv[0]) + 1 : 0;
+        snprintf(beside, sizeof beside, "%.*stuning_mi355x.json", dir, argv[0]);
+        FILE *f = fopen(beside, "r");
+        if (f) {
+            fclose(f);
+            tuning = beside;
+        }
+    }
+    if (tuning && (rc = ntt_engine_autotune_load(tuning)) != 0) {
         fprintf(stderr, "compute_e: %s\n", ntt_engine_last_error());
         return 1;
     }

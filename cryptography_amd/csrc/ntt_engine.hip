@@ -10,6 +10,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -133,6 +134,13 @@ int engine_init(int device)
     g.gc.inv012p = h_shoup(g.gc.inv012, p2);
     g.init = 1;
     g_generation++;
+    /* a tuning file named by the environment (as ntt_engine_autotune_save writes it) */
+    if (const char *tf = getenv("NTT_AMD_TUNING")) {
+        if (*tf) {
+            const int rc = ntt_engine_autotune_load(tf);
+            if (rc) return rc;
+        }
+    }
     return NTT_OK;
 }
 

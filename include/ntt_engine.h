@@ -36,7 +36,9 @@ const char *ntt_engine_last_error(void);
 
 /* Initialise on HIP device `device` (idempotent).  Builds the twiddle and
  * Shoup tables on the device.  No counterpart needed by callers of the
- * reference API: ntt_mul_init calls it. */
+ * reference API: ntt_mul_init calls it.  When the environment variable
+ * NTT_AMD_TUNING names a tuning file (ntt_engine_autotune_save's format), it
+ * is loaded here; a file that cannot be read or parsed fails the init. */
 int ntt_engine_init(int device);
 /* Release every device buffer and the stream (cf. ntt_mul_teardown_impl). */
 void ntt_engine_teardown(void);
@@ -130,7 +132,12 @@ int ntt_engine_set_xtab(int on);
  * when max_log >= 24, the tile batch (1, 2 or 4) on the longest length, and
  * sets the fastest.  A few seconds.  save/load keep the result in a small
  * JSON file ({"threshold": T, "tile_batch": B, "pass_cap": {"<log_len>": cap,
- * ...}}); a file without "tile_batch" leaves the current value. */
+ * ...}, "crossover_us": [[n, schoolbook, ntt, karatsuba_lower_bound], ...]});
+ * the loader reads threshold, tile_batch and pass_cap (a file without
+ * tile_batch or pass_cap leaves the current value) and skips other keys.
+ * crossover_us records, per operand size n, microseconds per product of a
+ * batch of 256 n x n products on each algorithm and, as a lower bound for one
+ * Karatsuba level, three products of ceil(n/2) + 1 limbs on the faster one. */
 int ntt_engine_autotune(int min_log, int max_log);
 int ntt_engine_autotune_save(const char *path);
 int ntt_engine_autotune_load(const char *path);

@@ -261,6 +261,25 @@ def test_mul_ntt3_multithreaded_2_22_bits():
     assert O.orc().orc_log_len(len(a), len(b)) == 19
 
 
+@pytest.mark.parametrize("na,nb,threads", [(1, 1, 4), (3, 5, 2), (1000, 999, 8), (40000, 3, 4),
+                                           (1 << 16, (1 << 16) - 7, 8), (1 << 13, 1 << 13, 1)])
+def test_mul_ntt3_all_core_restatement(na, nb, threads):
+    """The all-core form of the restatement (the CPU baseline at full size:
+    cache-blocked small stages, threaded large ones, one twiddle table per
+    prime and direction) gives GMP's products and the sequential form's."""
+    a = O.splitmix(na * 31 + 1, na)
+    b = O.splitmix(nb * 37 + 2, nb)
+    got = O.mul_ntt3_par(a, b, threads)
+    assert np.array_equal(got, O.gmp_mul(a, b))
+    if na + nb <= 4096:
+        assert np.array_equal(got, O.mul_ntt3(a, b))
+
+
+def test_mul_ntt3_all_core_all_ones():
+    a = np.full(20000, 0xFFFFFFFFFFFFFFFF, np.uint64)
+    assert np.array_equal(O.mul_ntt3_par(a, a, 8), O.gmp_mul(a, a))
+
+
 # ---- product tree and e ------------------------------------------------------------
 
 def test_binary_split_matches_reference_cells():
