@@ -103,12 +103,13 @@ def has_xtab(logS, logT):
     return 3 <= logS <= 10 and 8 <= logS + logT <= 20
 
 
-def launch_table(L, logL, na, nb):
+def launch_table(L, logL, na, nb, square=False):
     """Per launch of one product (in launch order): rocprof kernel name, blocks, algorithmic bytes.
-    Algorithmic bytes: residues are 8 B, k = 3 primes; forward passes cover both operands, the first
-    one reads the operand limbs instead of residues; the fused middle pass reads both spectra and
-    writes one; inverse passes read and write one row set; Garner reads 3 residue rows and writes
-    the product limbs."""
+    Algorithmic bytes: residues are 8 B, k = 3 primes; forward passes cover both operands (one for a
+    square), the first one reads the operand limbs instead of residues; the fused middle pass reads
+    both spectra (one for a square) and writes one; inverse passes read and write one row set;
+    Garner reads 3 residue rows and writes the product limbs."""
+    nops = 1 if square else 2
     Lw = 1 << logL
     ntiles = max(1, Lw // 4096)
     plan = pass_plan(L, logL)
@@ -124,10 +125,11 @@ def launch_table(L, logL, na, nb):
     out, done = [], 0
     for i, s in enumerate(plan[:-1]):
         kind = 3 if i == 0 else (6 if has_xtab(s, logL - done - s) else 0)
-        byt = 8 * (na + nb) + 2 * K_PRIMES * 8 * Lw if i == 0 else 2 * K_PRIMES * 16 * Lw
-        out.append((f"k_fast_pass<{s}, {kind}>", blocks(kind) * K_PRIMES * 2, byt))
+        byt = 8 * (na + (0 if square else nb)) + nops * K_PRIMES * 8 * Lw if i == 0 else nops * K_PRIMES * 16 * Lw
+        out.append((f"k_fast_pass<{s}, {kind}>", blocks(kind) * K_PRIMES * nops, byt))
         done += s
-    out.append((f"k_fast_pass<{plan[-1]}, {2 if len(plan) > 1 else 4}>", ntiles * K_PRIMES, K_PRIMES * 24 * Lw))
+    out.append((f"k_fast_pass<{plan[-1]}, {2 if len(plan) > 1 else 4}>", ntiles * K_PRIMES,
+                K_PRIMES * (16 if square else 24) * Lw))
     for i in range(len(plan) - 2, -1, -1):
         done -= plan[i]
         logT = logL - done - plan[i]
@@ -140,14 +142,19 @@ def launch_table(L, logL, na, nb):
     return out
 
 
-def profile_traffic(name, blocks):
-    """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary
-    (profiles/mul_2p30_kernels.json: FETCH_SIZE and WRITE_SIZE, scaled by a copy calibration)."""
+def profile_entry(name, blocks, square=False):
+    """A launch shape's entry in the committed rocprofv3 summary (profiles/mul_2p30_kernels.json)."""
     path = os.path.join(ROOT, "profiles", "mul_2p30_kernels.json")
     if not os.path.exists(path):
-        return None
-    k = json.load(open(path))["kernels"].get(f"{name}|{blocks}")
-    if not k or k.get("fetch_bytes") is None or k.get("write_bytes") is None:
+        return {}
+    return json.load(open(path))["kernels"].get(f"{name}|{blocks}" + ("|sqr" if square else ""), {})
+
+
+def profile_traffic(name, blocks, square=False):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy calibration)."""
+    k = profile_entry(name, blocks, square)
+    if k.get("fetch_bytes") is None or k.get("write_bytes") is None:
         return None
     return k["fetch_bytes"] + k["write_bytes"]
 
@@ -214,6 +221,15 @@ def headline(args, E, O):
         E.sqr_device(dc, da, na, sync=False)
     E.engine.check(L.ntt_engine_sync())
     sq_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    # per-launch event timings of the square
+    L.ntt_engine_set_timing(1)
+    sq_ntt, sq_per = [], {}
+    for _ in range(max(3, args.steps // 2)):
+        E.sqr_device(dc, da, na)
+        sq_ntt.append(L.ntt_engine_last_ntt_ms())
+        for i, (nm, ms) in enumerate(kernel_times(L)):
+            sq_per.setdefault(f"{i}:{nm}", []).append(ms)
+    L.ntt_engine_set_timing(0)
     for x in (da, db, dc):
         x.close()
     # the same product through the host-buffer ABI (ntt_mul_limbs: operands in
@@ -244,8 +260,32 @@ def headline(args, E, O):
     roof["per_launch"] = [{"kernel": nm, "launch_ms": t, "alg_bytes": ab,
                            "achieved": ab / (t * 1e-3) / 1e9, "frac": ab / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
                           for (nm, _, ab), t in zip(table, times)]
+    # VALU instructions per butterfly of the dominant kernel (committed SQ counters)
+    if nbits == 1 << 30:
+        roof["valu_instr_per_butterfly"] = profile_entry(name, blocks).get("valu_per_butterfly")
+        roof["valu_instr_per_butterfly_by_kernel"] = {
+            nm: round(profile_entry(nm, bl).get("valu_per_butterfly") or 0, 1) for nm, bl, _ in table
+            if profile_entry(nm, bl).get("valu_per_butterfly")}
+    # the square: B_sqr = 64 k L against its NTT stages, and its dominant launch
+    b_sqr = 64 * K_PRIMES * Llen
+    sq_table = launch_table(L, logL, na, na, square=True)
+    sq_times = [float(np.median(v)) for v in sq_per.values()]
+    j_dom = int(np.argmax(sq_times[:len(sq_table)]))
+    sq_name, sq_blocks, sq_alg = sq_table[j_dom]
+    t_sq_ntt = float(np.median(sq_ntt))
+    sq_roof = {"bound": "hbm", "kernel": f"{sq_name} (launch {j_dom} of the square)",
+               "achieved": sq_alg / (sq_times[j_dom] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "traffic": profile_traffic(sq_name, sq_blocks, square=True) if nbits == 1 << 30 else None,
+               "alg_bytes_per_launch": sq_alg, "launch_ms": sq_times[j_dom],
+               "ntt_stages_ms": t_sq_ntt, "b_alg_bytes": b_sqr,
+               "ntt_stages_achieved": b_sqr / (t_sq_ntt * 1e-3) / 1e9}
+    sq_roof["frac"] = sq_roof["achieved"] / HBM_PEAK_GBS
+    sq_roof["ntt_stages_frac"] = sq_roof["ntt_stages_achieved"] / HBM_PEAK_GBS
+    sq_roof["per_launch"] = [{"kernel": nm, "launch_ms": t, "alg_bytes": ab,
+                              "frac": ab / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+                             for (nm, _, ab), t in zip(sq_table, sq_times)]
     return {"operands": (a, b), "product": want, "gmp_mul_s": t_gmp_mul, "gmp_sqr_s": t_gmp_sqr,
-            "ms_per_step": ms_per_step, "square_ms": sq_ms, "host_api_ms": float(np.median(host_ms)), "logL": logL, "Llen": Llen, "nbits": nbits,
+            "ms_per_step": ms_per_step, "square_ms": sq_ms, "square_roofline": sq_roof, "host_api_ms": float(np.median(host_ms)), "logL": logL, "Llen": Llen, "nbits": nbits,
             "kernels_ms": kern, "total_event_ms": float(np.median(tot_ms)), "roofline": roof}
 
 
@@ -611,6 +651,7 @@ def main():
                                f"3-prime CRT, NTT length 2^{h['logL']}, operands resident in HBM",
                    "operand_bits": h["nbits"], "transform_len": h["Llen"], "primes": K_PRIMES, "seed": [1, 2]},
         "square_ms": h["square_ms"],
+        "square_roofline": h["square_roofline"],
         "host_api_ms": h.get("host_api_ms"),
         "kernels_ms": h["kernels_ms"],
         "total_event_ms": h["total_event_ms"],

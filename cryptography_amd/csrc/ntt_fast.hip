@@ -84,6 +84,20 @@ __device__ __forceinline__ void lds_sync()
     asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+/* Ordering for LDS data that only the calling wave wrote and reads: its LDS
+ * operations complete in issue order, so waiting for them (and keeping the
+ * compiler from moving memory accesses across) is enough; no block barrier. */
+__device__ __forceinline__ void wave_lds_sync()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+}
+template <bool WL>
+__device__ __forceinline__ void round_sync()
+{
+    if constexpr (WL) wave_lds_sync();
+    else lds_sync();
+}
+
 /* XCD-aware tile order: consecutive blocks go to different XCDs (round robin
  * over 8), so tile numbers are dealt out in 8 contiguous ranges, one per XCD;
  * neighbouring column tiles then share their 128-byte lines in one L2. */
@@ -812,25 +826,26 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
 }
 
 /* DIF rounds of an S = 2^LOGS transform except the last (smallest) one, and
- * DIT rounds except the first: the fused middle pass does that round itself. */
-template <int PI, int LOGS, int LH, int RLL>
+ * DIT rounds except the first: the fused middle pass does that round itself.
+ * WL: the rounds are wave-local (see mid_tile) and need no block barrier. */
+template <int PI, int LOGS, int LH, int RLL, bool WL = false>
 __device__ __forceinline__ void dif_upper(uint64_t *lds, const TwPair *tw)
 {
     constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
     if constexpr (LH - RL >= 0) {
         dif_round<PI, LOGS, LH, RL, false, RLL>(lds, tw, 0, XtwCtx{});
-        lds_sync();
-        dif_upper<PI, LOGS, LH - RL, RLL>(lds, tw);
+        round_sync<WL>();
+        dif_upper<PI, LOGS, LH - RL, RLL, WL>(lds, tw);
     }
 }
-template <int PI, int LOGS, int LH, int RLL>
+template <int PI, int LOGS, int LH, int RLL, bool WL = false>
 __device__ __forceinline__ void dit_upper(uint64_t *lds, const TwPair *tw)
 {
     constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
     if constexpr (LH - RL >= 0) {
-        dit_upper<PI, LOGS, LH - RL, RLL>(lds, tw);
+        dit_upper<PI, LOGS, LH - RL, RLL, WL>(lds, tw);
         dit_round<PI, LOGS, LH, RL, false, RLL>(lds, tw, 0, XtwCtx{});
-        lds_sync();
+        round_sync<WL>();
     }
 }
 
@@ -869,6 +884,14 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     constexpr int NG = (FT_SIZE >> RLL) / FTPB;      /* its groups per thread */
     constexpr int LNB = LOGS - RLL;
     static_assert(NG * RR == FT_PER, "group layout");
+    /* Wave-local rounds: with S = 2^LOGS, LOGS a multiple of 3 (radix-8
+     * rounds only) and one column (T = 1), wave w's groups in every round lie
+     * in tile words [512w, 512w + 512), and so do the tile transfers it
+     * issues (its four 1 KB chunks) and, below, the words it stores.  After
+     * the stage twiddles are written, no wave reads what another wrote, so
+     * the rounds need no block barrier, only the wave's own LDS wait. */
+    constexpr bool WL = !FIRST && LOGS % 3 == 0;
+    static_assert(!WL || (FT_SIZE / 128 / (FTPB / 64)) * 128 == 512, "a wave's tile chunks are its 512 words");
     FGeom g{0, 0, pa.logG};
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     uint64_t ra[FT_PER];
@@ -886,8 +909,8 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
         stage_twiddles_mid<PI, LOGS, RLL>(pa.small_tw, tw);
         WAIT_VMCNT(0);
     }
-    lds_sync();
-    dif_upper<PI, LOGS, LOGS - 1, RLL>(lds, tw);
+    lds_sync();      /* the stage twiddles (and, if not wave-local, a's tile) */
+    dif_upper<PI, LOGS, LOGS - 1, RLL, WL>(lds, tw);
     /* a's smallest round, kept in registers: its groups leave LDS first */
 #pragma unroll
     for (int n = 0; n < NG; n++) {
@@ -901,7 +924,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
      * blocks share a CU instead of three, which was measured slower); the
      * transfer is in flight during a's register round */
     if (!pa.square) {
-        lds_sync();
+        round_sync<WL>();   /* a's reads are done before b's tile overwrites them */
         if constexpr (FIRST) {
 #pragma unroll
             for (int k = 0; k < FT_PER; k++) {
@@ -925,8 +948,8 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     for (int i = 0; i < FT_PER; i++) asm volatile("" : "+v"(ra[i]));
     if (!pa.square) {
         if constexpr (!FIRST) WAIT_VMCNT(0);
-        lds_sync();
-        dif_upper<PI, LOGS, LOGS - 1, RLL>(lds, tw);
+        round_sync<WL>();
+        dif_upper<PI, LOGS, LOGS - 1, RLL, WL>(lds, tw);
     }
     const uint64_t s = pa.scale[PI], sp = pa.scale_p[PI];
 #pragma unroll
@@ -953,11 +976,22 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 #pragma unroll
         for (int i = 0; i < RR; i++) lds[LP(base + i)] = x[i];
     }
-    lds_sync();
-    dit_upper<PI, LOGS, LOGS - 1, RLL>(lds, tw);
-    tile_span<LOGS>(pa, g, t, threadIdx.x, base, step);
+    round_sync<WL>();
+    dit_upper<PI, LOGS, LOGS - 1, RLL, WL>(lds, tw);
+    if constexpr (WL) {
+        /* wave w stores its own words 512w + lane + 64k: 512-byte runs (T = 1,
+         * so the tile is contiguous in its row, pads included) */
+        const uint32_t q0 = ((threadIdx.x >> 6) << 9) + (threadIdx.x & 63);
+        uint32_t m, r;
+        base = res_at(ft_addr(LOGS, g, t, q0, m, r), pa.pad_log);
+        step = res_at(ft_addr(LOGS, g, t, q0 + 64, m, r), pa.pad_log) - base;
 #pragma unroll
-    for (int k = 0; k < FT_PER; k++) xa[base + k * step] = lds[LP(threadIdx.x + k * FTPB)];
+        for (int k = 0; k < FT_PER; k++) xa[base + k * step] = lds[LP(q0 + 64 * k)];
+    } else {
+        tile_span<LOGS>(pa, g, t, threadIdx.x, base, step);
+#pragma unroll
+        for (int k = 0; k < FT_PER; k++) xa[base + k * step] = lds[LP(threadIdx.x + k * FTPB)];
+    }
 }
 
 #define TW_LDS_MAX 1024  /* stage-twiddle entries per direction held in LDS (S - 1) */

@@ -2,7 +2,6 @@
 
 Every call goes through the C ABI of libntt_amd.so.  Results must be bit-exact.
 """
-import ctypes
 import random
 
 import numpy as np
