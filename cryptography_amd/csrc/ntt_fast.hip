@@ -399,8 +399,11 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
 }
 
 /* REDUCEDIN: the smallest round's (LD = 0) inputs are below 2p. */
-template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0, bool REDUCEDIN = false>
-__device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint32_t logC, const XtwCtx &xc)
+/* GSTORE: the round's outputs go to global memory instead of back to LDS,
+ * tile word q to gout[q] (a tile contiguous in its row: one column, T = 1). */
+template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0, bool REDUCEDIN = false, bool GSTORE = false>
+__device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint32_t logC, const XtwCtx &xc,
+                                          uint64_t *gout = nullptr)
 {
     constexpr int R = 1 << RL;
     constexpr int LD = LH - (RL - 1);
@@ -447,8 +450,13 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
                 }
             }
         }
+        if constexpr (GSTORE) {
 #pragma unroll
-        for (int i = 0; i < R; i++) lds[LP(base + ((uint32_t)(i << LD) << logC))] = x[i];
+            for (int i = 0; i < R; i++) gout[base + ((uint32_t)(i << LD) << logC)] = x[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < R; i++) lds[LP(base + ((uint32_t)(i << LD) << logC))] = x[i];
+        }
     }
 }
 
@@ -838,14 +846,15 @@ __device__ __forceinline__ void dif_upper(uint64_t *lds, const TwPair *tw)
         dif_upper<PI, LOGS, LH - RL, RLL, WL>(lds, tw);
     }
 }
-template <int PI, int LOGS, int LH, int RLL, bool WL = false>
-__device__ __forceinline__ void dit_upper(uint64_t *lds, const TwPair *tw)
+/* GSTORE: the last (top) round writes the tile to gout (see dit_round). */
+template <int PI, int LOGS, int LH, int RLL, bool WL = false, bool GSTORE = false>
+__device__ __forceinline__ void dit_upper(uint64_t *lds, const TwPair *tw, uint64_t *gout = nullptr)
 {
     constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
     if constexpr (LH - RL >= 0) {
-        dit_upper<PI, LOGS, LH - RL, RLL, WL>(lds, tw);
-        dit_round<PI, LOGS, LH, RL, false, RLL>(lds, tw, 0, XtwCtx{});
-        round_sync<WL>();
+        dit_upper<PI, LOGS, LH - RL, RLL, WL, false>(lds, tw);
+        dit_round<PI, LOGS, LH, RL, false, RLL, false, GSTORE>(lds, tw, 0, XtwCtx{}, gout);
+        if constexpr (!GSTORE) round_sync<WL>();
     }
 }
 
@@ -977,10 +986,15 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
         for (int i = 0; i < RR; i++) lds[LP(base + i)] = x[i];
     }
     round_sync<WL>();
-    dit_upper<PI, LOGS, LOGS - 1, RLL, WL>(lds, tw);
-    if constexpr (WL) {
-        /* wave w stores its own words 512w + lane + 64k: 512-byte runs (T = 1,
-         * so the tile is contiguous in its row, pads included) */
+    if constexpr (WL && LOGS > RLL) {
+        /* the top inverse round stores straight from registers (T = 1: the
+         * tile is contiguous in its row, pads fall between tiles; for
+         * S = 512, lane l of wave w holds words 512w + l + 64k, 512-byte runs) */
+        uint32_t m, r;
+        dit_upper<PI, LOGS, LOGS - 1, RLL, true, true>(lds, tw, xa + res_at(ft_addr(LOGS, g, t, 0, m, r), pa.pad_log));
+    } else if constexpr (WL) {
+        /* one round only (S = 8), done above: wave w stores its own words
+         * 512w + lane + 64k from LDS */
         const uint32_t q0 = ((threadIdx.x >> 6) << 9) + (threadIdx.x & 63);
         uint32_t m, r;
         base = res_at(ft_addr(LOGS, g, t, q0, m, r), pa.pad_log);
@@ -988,6 +1002,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 #pragma unroll
         for (int k = 0; k < FT_PER; k++) xa[base + k * step] = lds[LP(q0 + 64 * k)];
     } else {
+        dit_upper<PI, LOGS, LOGS - 1, RLL>(lds, tw);
         tile_span<LOGS>(pa, g, t, threadIdx.x, base, step);
 #pragma unroll
         for (int k = 0; k < FT_PER; k++) xa[base + k * step] = lds[LP(threadIdx.x + k * FTPB)];
