@@ -66,7 +66,6 @@ struct Engine {
     double last_ntt_ms = 0, last_total_ms = 0;
     int timing_pending = 0;
     GarnerConsts gc;
-    char err[512] = "";
 };
 
 extern Engine g;

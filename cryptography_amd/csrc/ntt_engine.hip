@@ -54,11 +54,16 @@ hipError_t mark(const char *name)
     return hipEventRecord(g.ev[g.n_marks++], g.stream);
 }
 
+/* The message of the calling thread's last error: callers on several threads
+ * (serving submitters, the scheduler thread, bigint_mul from OpenMP regions)
+ * each read their own. */
+static thread_local char t_err[512] = "";
+
 int fail(int code, const char *fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
-    vsnprintf(g.err, sizeof g.err, fmt, ap);
+    vsnprintf(t_err, sizeof t_err, fmt, ap);
     va_end(ap);
     return code;
 }
@@ -860,7 +865,7 @@ using namespace ntte;
 
 extern "C" {
 
-const char *ntt_engine_last_error(void) { return g.err; }
+const char *ntt_engine_last_error(void) { return t_err; }
 
 int ntt_engine_init(int device)
 {

@@ -31,7 +31,7 @@ typedef enum {
     NTT_ERR_NOMEM = 6         /* device or host allocation failed */
 } ntt_status;
 
-/* Message for the last error on this engine (never NULL). */
+/* Message of the last error on the calling thread (never NULL). */
 const char *ntt_engine_last_error(void);
 
 /* Initialise on HIP device `device` (idempotent).  Builds the twiddle and
