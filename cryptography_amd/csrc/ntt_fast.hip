@@ -502,6 +502,88 @@ __device__ __forceinline__ void dif_round_hz(uint64_t *lds, const TwPair *twf, u
     }
 }
 
+/* DIF rounds of an S = 2^LOGS transform from half-size 2^LH down, except the
+ * smallest (LD = 0) round. */
+template <int PI, int LOGS, int LH>
+__device__ __forceinline__ void dif_above_last(uint64_t *lds, const TwPair *twf, uint32_t logC)
+{
+    constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
+    if constexpr (LH - RL >= 0) {
+        dif_round<PI, LOGS, LH, RL, false>(lds, twf, logC, XtwCtx{});
+        lds_sync();
+        dif_above_last<PI, LOGS, LH - RL>(lds, twf, logC);
+    }
+}
+
+/* Thread tid's group in the smallest DIF round of a radix-8 transform
+ * (LOGS a multiple of 3: one 8-point group per thread): rows m0 .. m0+7 of
+ * tile column c, at tile words q0 + i*C; the same rows go to gb + i*gs in
+ * the row. */
+template <int LOGS>
+__device__ __forceinline__ uint32_t last_group(const PassArgs &pa, const FGeom &g, uint32_t tt, uint32_t &c,
+                                               uint32_t &m0, size_t &gb, size_t &gs)
+{
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    c = tid & ((1u << g.logC) - 1);
+    const uint32_t u = tid >> g.logC;
+    m0 = (u & ((1u << (LOGS - 3)) - 1)) << 3;
+    const uint32_t q0 = ((((u >> (LOGS - 3)) << LOGS) + m0) << g.logC) + c;
+    uint32_t m, r;
+    gb = res_at(ft_addr(LOGS, g, tt, q0, m, r), pa.pad_log);
+    gs = res_at(ft_addr(LOGS, g, tt, q0 + (1u << g.logC), m, r), pa.pad_log) - gb;
+    return q0;
+}
+
+/* The smallest DIF round with the cross-twiddle chain (A, B looked up at
+ * tile start), stored from registers: no LDS round trip and no block
+ * barrier before the tile leaves (measured on the first pass of a 2^30-bit
+ * product: 3.89 -> 3.43 ms, and 78 -> 70 VGPRs). */
+template <int PI, int LOGS>
+__device__ __forceinline__ void dif_last_store(const PassArgs &pa, const FGeom &g, uint32_t tt, const uint64_t *lds,
+                                               uint64_t *x, const XtwCtx &xc)
+{
+    uint32_t c, m0;
+    size_t gb, gs;
+    const uint32_t q0 = last_group<LOGS>(pa, g, tt, c, m0, gb, gs);
+    uint64_t v[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) v[i] = lds[LP(q0 + ((uint32_t)i << g.logC))];
+    const_dif<PI, 3, 0>(v);
+    chain_apply<PI, 3>(v, xc.ta, xc.tb);
+#pragma unroll
+    for (int i = 0; i < 8; i++) x[gb + i * gs] = v[i];
+}
+
+/* The top DIT round of a C * S = 4096 tile (radix 8, LOGS a multiple of 3),
+ * stored from registers: thread tid's group is tile words tid + k*FTPB,
+ * which tile_span places at gb + k*gs. */
+template <int PI, int LOGS>
+__device__ __forceinline__ void dit_top_store(const uint64_t *lds, const TwPair *twi, uint32_t logC, uint64_t *x,
+                                              size_t gb, size_t gs)
+{
+    constexpr int LH = LOGS - 1, RL = 3, R = 8, LD = LH - (RL - 1);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t j = tid >> logC;            /* the group's first row, below S/8 = 2^LD */
+    uint64_t v[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) v[i] = lds[LP(tid + i * FTPB)];
+#pragma unroll
+    for (int s = RL - 1; s >= 0; s--) {
+        const int half = R >> (s + 1);
+        const int lhs = LH - s;
+#pragma unroll
+        for (int i = 0; i < R; i++) {
+            if (i & half) continue;
+            const uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
+            const TwPair w = twi[(1u << lhs) - 1 + e];
+            dit_bfly<PI>(v[i], v[i + half], w.w, w.wp);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) x[gb + i * gs] = v[i];
+}
+
 /* All DIF rounds of an S = 2^LOGS transform, top stage first, radix 8 while
  * three or more stages remain. */
 template <int PI, int LOGS, int LH, bool XT = false, bool LAZYLAST = false>
@@ -702,6 +784,43 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
     if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false>(xc, g.logC);
     if constexpr (!FIRST) WAIT_VMCNT(0);
     lds_sync();
+    if constexpr (XTAB && !FIRST && LOGS % 3 == 0 && LOGS >= 6) {
+        /* The smallest round stores from registers, and the next tile's
+         * transfer starts once every thread has read its group, so it is in
+         * flight during that round, the table products and the stores
+         * (3.51 -> 3.43 ms at L = 2^27, two tiles per block).  The table
+         * entries of a group are rows m0 .. m0+7 of its column. */
+        const uint64_t *xt = pa.xtab + ((size_t)PI << (LOGS + g.logT));
+        for (;;) {
+            dif_above_last<PI, LOGS, LOGS - 1>(lds, twf, g.logC);
+            uint32_t c, m0;
+            size_t gb, gs;
+            const uint32_t q0 = last_group<LOGS>(pa, g, tt, c, m0, gb, gs);
+            uint64_t v[8], w[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) v[i] = lds[LP(q0 + ((uint32_t)i << g.logC))];
+#pragma unroll
+            for (int i = 0; i < 8; i++) w[i] = xt[((size_t)(m0 + i) << g.logT) + xc.col0 + c];
+            const bool more = t + 1 < tend;
+            if (more) {
+                tt = xtab_tile(++t, colTiles, pa.ntiles);
+                lds_sync();       /* every group read: the tile buffer is free */
+                tile_dma<LOGS>(pa, x, g, tt, lds);
+            }
+            /* the last stage in [0, 4p): the Montgomery product by a table
+             * entry below p returns (0, 2p) */
+            const_dif<PI, 3, 0, true>(v);
+#pragma unroll
+            for (int i = 0; i < 8; i++) x[gb + i * gs] = mont_mul_lazy<PI>(v[i], w[i]);
+            if (!more) return;
+            xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
+            WAIT_VMCNT(0);
+            lds_sync();
+        }
+    }
+    /* the first pass of a radix-8 transform stores its smallest round from
+     * registers */
+    constexpr bool REGSTORE = FIRST && LOGS % 3 == 0 && LOGS >= 6 && xt_ahead<LOGS>();
     for (;;) {
         uint32_t tid;
         if constexpr (FIRST) {
@@ -715,7 +834,12 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
             if (pa.half_zero) dif_round_hz<PI, LOGS>(lds, twf, g.logC);   /* a bare twiddle product on top */
             else dif_round<PI, LOGS, LOGS - 1, 3, false>(lds, twf, g.logC, xc);
             lds_sync();
-            dif_all<PI, LOGS, LOGS - 4, true>(lds, twf, g.logC, xc);
+            if constexpr (REGSTORE) {
+                dif_above_last<PI, LOGS, LOGS - 4>(lds, twf, g.logC);
+                dif_last_store<PI, LOGS>(pa, g, tt, lds, x, xc);
+            } else {
+                dif_all<PI, LOGS, LOGS - 4, true>(lds, twf, g.logC, xc);
+            }
         } else {
             /* chain: cross twiddles in the last round; table: the last stage
              * stays in [0, 4p), as the store's Montgomery product by a table
@@ -724,7 +848,7 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
         }
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
-        {
+        if constexpr (!REGSTORE) {
             size_t base, step;
             tile_span<LOGS>(pa, g, tt, tid, base, step);
             if constexpr (XTAB) {
@@ -811,10 +935,20 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
         /* otherwise the inverse cross twiddles go into the first round; the
          * table's Montgomery products (an entry below p) leave the tile
          * below 2p */
-        dit_all<PI, LOGS, LOGS - 1, !LOADCHAIN, XTAB>(lds, twi, g.logC, xc);
-        tid = threadIdx.x;
-        asm volatile("" : "+v"(tid));
-        {
+        if constexpr (LOADCHAIN && LOGS % 3 == 0 && LOGS >= 6) {
+            /* radix-8 rounds: the top round stores from registers (the
+             * inverse passes of a 2^30-bit product: 1.89 -> 1.83 ms and
+             * 2.14 -> 2.11 ms) */
+            dit_all<PI, LOGS, LOGS - 4, false, XTAB>(lds, twi, g.logC, xc);
+            tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            size_t base, step;
+            tile_span<LOGS>(pa, g, tt, tid, base, step);
+            dit_top_store<PI, LOGS>(lds, twi, g.logC, x, base, step);
+        } else {
+            dit_all<PI, LOGS, LOGS - 1, !LOADCHAIN, XTAB>(lds, twi, g.logC, xc);
+            tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
             size_t base, step;
             tile_span<LOGS>(pa, g, tt, tid, base, step);
 #pragma unroll
