@@ -89,12 +89,11 @@ def b_alg(logL):
 
 
 def pass_plan(L, logL):
-    """The engine's pass split of a length-2^logL transform (plan_passes in ntt_engine.hip)."""
-    L.ntt_engine_get_pass_cap.argtypes = [ctypes.c_int]
-    cap = L.ntt_engine_get_pass_cap(logL)
-    npass = max(1, -(-logL // cap))
-    base, extra = divmod(logL, npass)
-    return [base + (1 if i >= npass - extra else 0) for i in range(npass)]
+    """The engine's pass split of a length-2^logL transform (ntt_engine_get_plan)."""
+    L.ntt_engine_get_plan.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    out = (ctypes.c_int * 8)()
+    n = L.ntt_engine_get_plan(logL, out, 8)
+    return [out[i] for i in range(n)]
 
 
 def has_xtab(logS, logT):

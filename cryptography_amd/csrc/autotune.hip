@@ -2,8 +2,9 @@
  * autotune.hip — measures and sets the engine's tunable parameters:
  *   - the schoolbook / NTT crossover (limbs per operand), timed on batches of
  *     equal-size products with each algorithm;
- *   - the pass split of every transform length (the largest sub-transform per
- *     pass, 2^8 .. 2^10), timed on one product per length;
+ *   - the pass split of every transform length: every split into the fewest
+ *     passes of at most 2^10 points (up to 2^20, also one pass more), each
+ *     pass 2^3 points or more, timed on products enqueued back to back;
  *   - the tile shape of the long passes (tiles per block), timed on the
  *     longest length.
  * Results can be saved to and loaded from a small JSON file.
@@ -17,6 +18,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <functional>
 #include <vector>
 
 #include "engine_internal.h"
@@ -74,6 +76,14 @@ struct CrossSample {
     double t_sb, t_ntt, t_kara_lb;
 };
 std::vector<CrossSample> g_cross;
+
+/* Split measurements of the last autotune run: per length, every pass split
+ * tried and its seconds per product. */
+struct SplitSamples {
+    int logL;
+    std::vector<std::pair<std::vector<int>, double>> cand;
+};
+std::vector<SplitSamples> g_tune;
 
 /* ---- a small JSON reader for the tuning file --------------------------------
  * Values: numbers, strings, objects and arrays; the keys the engine reads
@@ -159,7 +169,31 @@ int ntt_engine_autotune_save(const char *path)
             fprintf(f, "%s\"%d\": %d", first ? "" : ", ", l, g.max_logs[l]);
             first = 0;
         }
+    fprintf(f, "}, \"pass_split\": {");
+    first = 1;
+    for (int l = 1; l <= NTT_MAX_LOG_LEN; l++)
+        if (g.split_n[l]) {
+            fprintf(f, "%s\"%d\": [", first ? "" : ", ", l);
+            for (int i = 0; i < g.split_n[l]; i++) fprintf(f, "%s%d", i ? ", " : "", g.split[l][i]);
+            fprintf(f, "]");
+            first = 0;
+        }
     fprintf(f, "}");
+    if (!g_tune.empty()) {
+        /* microseconds per product of every split tried, for the record */
+        fprintf(f, ", \"split_us\": {");
+        for (size_t i = 0; i < g_tune.size(); i++) {
+            fprintf(f, "%s\"%d\": [", i ? ", " : "", g_tune[i].logL);
+            for (size_t k = 0; k < g_tune[i].cand.size(); k++) {
+                const auto &c = g_tune[i].cand[k];
+                fprintf(f, "%s[[", k ? ", " : "");
+                for (size_t j = 0; j < c.first.size(); j++) fprintf(f, "%s%d", j ? ", " : "", c.first[j]);
+                fprintf(f, "], %.3f]", c.second * 1e6);
+            }
+            fprintf(f, "]");
+        }
+        fprintf(f, "}");
+    }
     if (!g_cross.empty()) {
         /* microseconds per product, for the record (the loader skips it) */
         fprintf(f, ", \"crossover_us\": [");
@@ -174,9 +208,10 @@ int ntt_engine_autotune_save(const char *path)
 }
 
 /* Reader for the file written above: {"threshold": T, "tile_batch": B,
- * "pass_cap": {"<log_len>": cap, ...}, ...}; other keys are skipped, a
- * missing tile_batch or pass_cap leaves the current value, and anything
- * malformed or out of range is an error that changes nothing. */
+ * "pass_cap": {"<log_len>": cap, ...}, "pass_split": {"<log_len>": [s0, s1,
+ * ...], ...}, ...}; other keys are skipped, a missing tile_batch, pass_cap or
+ * pass_split leaves the current value, and anything malformed or out of range
+ * is an error that changes nothing. */
 int ntt_engine_autotune_load(const char *path)
 {
     std::lock_guard<std::recursive_mutex> lk(g_lock);
@@ -188,8 +223,11 @@ int ntt_engine_autotune_load(const char *path)
     buf[n] = 0;
     JsonIn in{buf.data()};
     int thr = -1, tb = g.tile_batch;
-    uint8_t caps[NTT_MAX_LOG_LEN + 1];
+    uint8_t caps[NTT_MAX_LOG_LEN + 1], sn[NTT_MAX_LOG_LEN + 1], sp[NTT_MAX_LOG_LEN + 1][8];
     memcpy(caps, g.max_logs, sizeof caps);
+    memcpy(sn, g.split_n, sizeof sn);
+    memcpy(sp, g.split, sizeof sp);
+    bool have_caps = false, have_split = false;
     if (!in.eat('{')) return fail(NTT_ERR_ARG, "%s: not a JSON object", path);
     if (!in.eat('}')) {
         do {
@@ -200,7 +238,34 @@ int ntt_engine_autotune_load(const char *path)
                 if (in.num(v)) thr = (int)v;
             } else if (!strcmp(key, "tile_batch")) {
                 if (in.num(v)) tb = (int)v;
+            } else if (!strcmp(key, "pass_split")) {
+                have_split = true;
+                memset(sn, 0, sizeof sn);
+                if (!in.eat('{')) { in.ok = false; break; }
+                if (!in.eat('}')) {
+                    do {
+                        char lk_[16];
+                        if (!in.str(lk_, sizeof lk_) || !in.eat(':') || !in.eat('[')) { in.ok = false; break; }
+                        const int l = atoi(lk_);
+                        if (l < 1 || l > NTT_MAX_LOG_LEN) return fail(NTT_ERR_ARG, "%s: pass split for 2^%d", path, l);
+                        int np = 0, sum = 0;
+                        do {
+                            if (!in.num(v)) break;
+                            const int ls = (int)v;
+                            if (np >= 8 || ls < 1 || ls > FT_MAX_LOGS || ls != v)
+                                return fail(NTT_ERR_ARG, "%s: pass split for 2^%d out of range", path, l);
+                            sp[l][np++] = (uint8_t)ls;
+                            sum += ls;
+                        } while (in.ok && in.eat(','));
+                        if (!in.ok || !in.eat(']')) { in.ok = false; break; }
+                        if (np == 0 || sum != l)
+                            return fail(NTT_ERR_ARG, "%s: pass split for 2^%d sums to 2^%d", path, l, sum);
+                        sn[l] = (uint8_t)np;
+                    } while (in.ok && in.eat(','));
+                    if (in.ok && !in.eat('}')) in.ok = false;
+                }
             } else if (!strcmp(key, "pass_cap")) {
+                have_caps = true;
                 memset(caps, 0, sizeof caps);
                 if (!in.eat('{')) { in.ok = false; break; }
                 if (!in.eat('}')) {
@@ -227,11 +292,18 @@ int ntt_engine_autotune_load(const char *path)
     g.threshold = thr;
     g.tile_batch = tb;
     memcpy(g.max_logs, caps, sizeof caps);
+    /* the latest setting decides a length's plan: new caps clear the splits
+     * of the lengths they list, then the file's splits apply */
+    if (have_caps)
+        for (int l = 1; l <= NTT_MAX_LOG_LEN; l++)
+            if (caps[l]) sn[l] = have_split ? sn[l] : 0;
+    memcpy(g.split_n, sn, sizeof sn);
+    memcpy(g.split, sp, sizeof sp);
     return NTT_OK;
 }
 
-/* Measure and set the crossover, the pass caps for transform lengths
- * 2^min_log .. 2^max_log and (max_log >= 24) the tile batch.  Takes a few
+/* Measure and set the crossover, the pass splits for transform lengths
+ * 2^min_log .. 2^max_log and (max_log >= 24) the tile batch.  Takes some
  * seconds. */
 int ntt_engine_autotune(int min_log, int max_log)
 {
@@ -266,33 +338,67 @@ int ntt_engine_autotune(int min_log, int max_log)
         g_cross.push_back({n, t_sb, t_ntt, 3 * std::min(t_h_sb, t_h_ntt)});
     }
     g.threshold = rc ? saved : best_thr;
-    /* 2. pass caps: one product of L/8 x L/8 limbs per length */
+    /* 2. pass splits: every split of length 2^l into the fewest passes of at
+     * most 2^10 points (and, up to 2^20, one pass more), each of 2^3 or more,
+     * timed on products of L/8 x L/8 limbs enqueued back to back */
+    g_tune.clear();
     for (int l = min_log; l <= max_log && !rc; l++) {
         size_t n = (size_t)1 << (l - 3);
         if (4 * n > words) break;
+        const int np0 = (l + FT_MAX_LOGS - 1) / FT_MAX_LOGS;
+        std::vector<std::vector<int>> cands;
+        for (int np = np0; np <= (l <= 20 ? np0 + 1 : np0); np++) {
+            std::vector<int> v(np, 3);
+            /* compositions of l into np parts in [3, 10], in lexicographic order */
+            std::function<void(int, int)> rec = [&](int i, int left) {
+                if (i == np - 1) {
+                    if (left >= 3 && left <= FT_MAX_LOGS) {
+                        v[i] = left;
+                        cands.push_back(v);
+                    }
+                    return;
+                }
+                for (int x = 3; x <= FT_MAX_LOGS && x <= left - 3 * (np - 1 - i); x++) {
+                    v[i] = x;
+                    rec(i + 1, left - x);
+                }
+            };
+            rec(0, l);
+        }
+        if (cands.size() < 2) continue;
+        const int k = (int)std::min<size_t>(64, std::max<size_t>(1, (size_t)1 << std::max(0, 24 - l)));
+        const uint8_t saved_n = g.split_n[l];
+        uint8_t saved_s[8];
+        memcpy(saved_s, g.split[l], 8);
         double best = 1e30;
-        int best_cap = FT_MAX_LOGS;
-        int last_npass = -1;
-        for (int cap = FT_MAX_LOGS; cap >= 8; cap--) {
-            int npass = (l + cap - 1) / cap;
-            if (npass == last_npass) continue;   /* same plan as a larger cap */
-            last_npass = npass;
-            g.max_logs[l] = (uint8_t)cap;
+        int best_i = -1;
+        SplitSamples rec_l{l, {}};
+        for (size_t ci = 0; ci < cands.size() && !rc; ci++) {
+            g.split_n[l] = (uint8_t)cands[ci].size();
+            for (size_t j = 0; j < cands[ci].size(); j++) g.split[l][j] = (uint8_t)cands[ci][j];
             std::vector<double> t;
             for (int r = 0; r < 4 && !rc; r++) {
                 double t0 = wall();
-                rc = enqueue_mul(buf + 2 * n, buf, n, buf + n, n, 0);
+                for (int q = 0; q < k && !rc; q++) rc = enqueue_mul(buf + 2 * n, buf, n, buf + n, n, 0);
                 if (!rc) rc = sync_check();
-                if (r) t.push_back(wall() - t0);
+                if (r) t.push_back((wall() - t0) / k);
             }
             if (rc) break;
             std::sort(t.begin(), t.end());
+            rec_l.cand.push_back({cands[ci], t[t.size() / 2]});
             if (t[t.size() / 2] < best) {
                 best = t[t.size() / 2];
-                best_cap = cap;
+                best_i = (int)ci;
             }
         }
-        g.max_logs[l] = (uint8_t)best_cap;
+        if (rc || best_i < 0) {
+            g.split_n[l] = saved_n;
+            memcpy(g.split[l], saved_s, 8);
+            break;
+        }
+        g.split_n[l] = (uint8_t)cands[best_i].size();
+        for (size_t j = 0; j < cands[best_i].size(); j++) g.split[l][j] = (uint8_t)cands[best_i][j];
+        g_tune.push_back(rec_l);
     }
     /* 3. tile shape: tiles per block of the forward / inverse passes, timed on
      * the longest transform.  A block takes more than one tile only when a

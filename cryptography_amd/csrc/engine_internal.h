@@ -43,6 +43,10 @@ struct Engine {
     /* largest sub-transform (log2) per pass, per transform length 2^l;
      * 0 = default (FT_MAX_LOGS).  Set by the autotuner. */
     uint8_t max_logs[NTT_MAX_LOG_LEN + 1] = {};
+    /* explicit pass splits (autotuned): split_n[l] passes of 2^split[l][i]
+     * points for transforms of length 2^l, first pass first; 0 = by max_logs */
+    uint8_t split_n[NTT_MAX_LOG_LEN + 1] = {};
+    uint8_t split[NTT_MAX_LOG_LEN + 1][8] = {};
     /* most tiles one block of a forward / inverse pass works through */
     int tile_batch = 1;
     /* cross-twiddle tables of passes whose S x T block has at most
@@ -98,6 +102,9 @@ struct PassPlan {
     int npass;
     uint32_t logS[8];
 };
+/* npass and every pass size of the plan in one word (a spectrum made with
+ * one plan is only read back under the same plan) */
+uint64_t plan_signature(int logL);
 
 /* Allocator whose resize() leaves trivially constructible elements
  * uninitialized: descriptor arrays of millions of entries are written in

@@ -171,14 +171,18 @@ void engine_teardown()
     for (int i = 0; i < NTT_MAX_MARKS; i++) (void)hipEventDestroy(g.ev[i]);
     (void)hipStreamDestroy(g.stream);
     int thr = g.threshold, tb = g.tile_batch, xt = g.use_xtab;
-    uint8_t ml[NTT_MAX_LOG_LEN + 1];
+    uint8_t ml[NTT_MAX_LOG_LEN + 1], sn[NTT_MAX_LOG_LEN + 1], sp[NTT_MAX_LOG_LEN + 1][8];
     memcpy(ml, g.max_logs, sizeof ml);
+    memcpy(sn, g.split_n, sizeof sn);
+    memcpy(sp, g.split, sizeof sp);
     g.~Engine();
     new (&g) Engine();
     g.threshold = thr;
     g.tile_batch = tb;
     g.use_xtab = xt;
     memcpy(g.max_logs, ml, sizeof ml);
+    memcpy(g.split_n, sn, sizeof sn);
+    memcpy(g.split, sp, sizeof sp);
 }
 
 /* Copies ordered after everything queued on the engine stream (the stream is
@@ -211,12 +215,25 @@ int log_len(size_t na, size_t nb)
 PassPlan plan_passes(int logL)
 {
     PassPlan pp{};
+    if (logL >= 1 && logL <= NTT_MAX_LOG_LEN && g.split_n[logL]) {
+        pp.npass = g.split_n[logL];
+        for (int i = 0; i < pp.npass; i++) pp.logS[i] = g.split[logL][i];
+        return pp;
+    }
     int cap = g.max_logs[logL] ? g.max_logs[logL] : FT_MAX_LOGS;
     pp.npass = (logL + cap - 1) / cap;
     if (pp.npass < 1) pp.npass = 1;
     int base = logL / pp.npass, extra = logL % pp.npass;
     for (int i = 0; i < pp.npass; i++) pp.logS[i] = base + (i >= pp.npass - extra ? 1 : 0);
     return pp;
+}
+
+uint64_t plan_signature(int logL)
+{
+    const PassPlan pp = plan_passes(logL);
+    uint64_t sig = (uint64_t)pp.npass;
+    for (int i = 0; i < pp.npass; i++) sig |= (uint64_t)pp.logS[i] << (4 + 4 * i);
+    return sig;
 }
 
 const uint64_t *xtab_for(int logS, int logT, hipStream_t st)
@@ -1064,6 +1081,7 @@ int ntt_engine_set_pass_cap(int log_len, int cap)
     if (log_len < 1 || log_len > NTT_MAX_LOG_LEN || (cap != 0 && (cap < 4 || cap > FT_MAX_LOGS)))
         return fail(NTT_ERR_ARG, "pass cap %d for 2^%d out of range", cap, log_len);
     g.max_logs[log_len] = (uint8_t)cap;
+    g.split_n[log_len] = 0;       /* the latest setting decides the plan */
     return NTT_OK;
 }
 
@@ -1071,6 +1089,34 @@ int ntt_engine_get_pass_cap(int log_len)
 {
     if (log_len < 1 || log_len > NTT_MAX_LOG_LEN) return 0;
     return g.max_logs[log_len] ? g.max_logs[log_len] : FT_MAX_LOGS;
+}
+
+int ntt_engine_set_pass_split(int log_len, const int *log_s, int npass)
+{
+    if (log_len < 1 || log_len > NTT_MAX_LOG_LEN) return fail(NTT_ERR_ARG, "pass split for 2^%d: no such length", log_len);
+    if (npass == 0) {
+        g.split_n[log_len] = 0;
+        return NTT_OK;
+    }
+    if (npass < 1 || npass > 8 || !log_s) return fail(NTT_ERR_ARG, "pass split for 2^%d: %d passes", log_len, npass);
+    int sum = 0;
+    for (int i = 0; i < npass; i++) {
+        if (log_s[i] < 1 || log_s[i] > FT_MAX_LOGS)
+            return fail(NTT_ERR_ARG, "pass split for 2^%d: a pass of 2^%d points", log_len, log_s[i]);
+        sum += log_s[i];
+    }
+    if (sum != log_len) return fail(NTT_ERR_ARG, "pass split for 2^%d sums to 2^%d", log_len, sum);
+    for (int i = 0; i < npass; i++) g.split[log_len][i] = (uint8_t)log_s[i];
+    g.split_n[log_len] = (uint8_t)npass;
+    return NTT_OK;
+}
+
+int ntt_engine_get_plan(int log_len, int *log_s, int max)
+{
+    if (log_len < 1 || log_len > NTT_MAX_LOG_LEN) return 0;
+    const PassPlan pp = plan_passes(log_len);
+    for (int i = 0; i < pp.npass && i < max; i++) log_s[i] = (int)pp.logS[i];
+    return pp.npass;
 }
 int ntt_engine_set_tile_batch(int tiles)
 {

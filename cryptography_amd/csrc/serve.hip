@@ -125,12 +125,16 @@ struct Spectrum {
     hipEvent_t ready = nullptr;   /* recorded after the transform that made it */
 };
 
+uint64_t spec_key(int logL) { return ((uint64_t)logL << 40) | plan_signature(logL); }
+
 struct Pinned {
     const uint64_t *ptr = nullptr;
     uint32_t n = 0;
     bool live = false;
     int users = 0;            /* queued or running requests that use it */
-    std::map<int, Spectrum> spec;   /* by log2 transform length */
+    /* by transform length and the plan it was made under (spec_key): a
+     * spectrum made before the length's pass plan changed is not read again */
+    std::map<uint64_t, Spectrum> spec;
 };
 
 struct Slot {
@@ -258,7 +262,7 @@ bool ntt_server::admit(Slot &s)
     const bool one_op = pin >= 0;
     /* memory the pinned operand's spectrum still needs (made with this batch) */
     size_t spec_need = 0;
-    if (one_op && !pins[pin].spec.count(pin_log))
+    if (one_op && !pins[pin].spec.count(spec_key(pin_log)))
         spec_need = round_up(NTT_NPRIMES * spectrum_rows_len(pin_log) * 8);
     const size_t room = arena.largest();
     std::vector<Req *> batch;
@@ -329,14 +333,15 @@ int ntt_server::launch(Slot &s, std::vector<Req *> &batch, size_t need)
     }
     Spectrum *sp = nullptr;
     if (pin >= 0) {
-        auto it = pins[pin].spec.find(logL);
+        const uint64_t key = spec_key(logL);
+        auto it = pins[pin].spec.find(key);
         if (it == pins[pin].spec.end()) {
             Spectrum fresh;
             fresh.bytes = NTT_NPRIMES * spectrum_rows_len(logL) * 8;
             fresh.mem = arena.alloc(fresh.bytes);
             if (!fresh.mem) return fail(NTT_ERR_NOMEM, "no arena block for a resident spectrum");
             HIP_TRY(hipEventCreateWithFlags(&fresh.ready, hipEventDisableTiming));
-            it = pins[pin].spec.emplace(logL, fresh).first;
+            it = pins[pin].spec.emplace(key, fresh).first;
             /* the pinned operand's forward passes, into the spectrum's rows */
             Workspace wsS;
             wsS.res = (uint64_t *)fresh.mem;

@@ -304,21 +304,37 @@ def compute_e(n_digits, batched_tree=True):
     return str(memoryview(buf)[:n_digits + 1], "ascii"), {k: getattr(st, k) for k, _ in EStats._fields_}
 
 
+def get_plan(log_len):
+    """The pass plan in effect for length 2^log_len: a tuple of pass sizes (log2), first pass first."""
+    L = lib()
+    L.ntt_engine_get_plan.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    out = (ctypes.c_int * 8)()
+    n = L.ntt_engine_get_plan(log_len, out, 8)
+    return tuple(out[i] for i in range(n))
+
+
+def set_pass_split(log_len, split):
+    """An explicit pass split for length 2^log_len (() clears it)."""
+    L = lib()
+    L.ntt_engine_set_pass_split.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    arr = (ctypes.c_int * max(1, len(split)))(*split)
+    check(L.ntt_engine_set_pass_split(log_len, arr, len(split)))
+
+
 def autotune(min_log=11, max_log=27, save=None, load=None):
-    """Run (or load) the autotuner; returns (threshold, {log_len: pass_cap})."""
+    """Run (or load) the autotuner; returns (threshold, {log_len: pass plan})."""
     L = lib()
     init()
     L.ntt_engine_autotune.argtypes = [ctypes.c_int, ctypes.c_int]
     L.ntt_engine_autotune_save.argtypes = [ctypes.c_char_p]
     L.ntt_engine_autotune_load.argtypes = [ctypes.c_char_p]
-    L.ntt_engine_get_pass_cap.argtypes = [ctypes.c_int]
     if load:
         check(L.ntt_engine_autotune_load(load.encode()))
     else:
         check(L.ntt_engine_autotune(min_log, max_log))
     if save:
         check(L.ntt_engine_autotune_save(save.encode()))
-    return L.ntt_engine_get_threshold(), {l: L.ntt_engine_get_pass_cap(l) for l in range(min_log, max_log + 1)}
+    return L.ntt_engine_get_threshold(), {l: get_plan(l) for l in range(min_log, max_log + 1)}
 
 
 def poly_mul(f, g, negacyclic=False):

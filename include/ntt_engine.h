@@ -113,9 +113,20 @@ int ntt_engine_get_threshold(void);
 void ntt_engine_set_threshold(int limbs);
 
 /* Largest sub-transform (log2, 4..10; 0 = default 10) of each pass for
- * transforms of length 2^log_len.  Set by the autotuner. */
+ * transforms of length 2^log_len, split into near-equal passes.  Setting it
+ * clears an explicit split of that length (below). */
 int ntt_engine_set_pass_cap(int log_len, int cap);
 int ntt_engine_get_pass_cap(int log_len);
+/* An explicit pass split for transforms of length 2^log_len: npass passes
+ * of 2^log_s[0], 2^log_s[1], ... points (first pass first; each 1..10, the
+ * sum log_len, npass <= 8).  It takes precedence over the pass cap; npass = 0
+ * clears it.  Set by the autotuner.  A change applies to later calls: batch
+ * graphs and product trees keep the plan they were captured with, and a
+ * server rebuilds a resident spectrum made under another plan. */
+int ntt_engine_set_pass_split(int log_len, const int *log_s, int npass);
+/* The plan in effect for length 2^log_len: writes up to max pass sizes
+ * (log2) to log_s and returns the number of passes (0 for a bad length). */
+int ntt_engine_get_plan(int log_len, int *log_s, int max);
 /* Most tiles (4096 residues each) one block of a forward or inverse pass
  * transforms in turn (default 1, range 1..64; applied to large transforms
  * only, small ones use 1). */
@@ -128,13 +139,18 @@ int ntt_engine_get_tile_batch(void);
 int ntt_engine_set_xtab(int on);
 
 /* Autotuner: times the schoolbook/NTT crossover on batches of equal-size
- * products, every pass split for lengths 2^min_log..2^max_log (11..28) and,
- * when max_log >= 24, the tile batch (1, 2 or 4) on the longest length, and
- * sets the fastest.  A few seconds.  save/load keep the result in a small
- * JSON file ({"threshold": T, "tile_batch": B, "pass_cap": {"<log_len>": cap,
- * ...}, "crossover_us": [[n, schoolbook, ntt, karatsuba_lower_bound], ...]});
- * the loader reads threshold, tile_batch and pass_cap (a file without
- * tile_batch or pass_cap leaves the current value) and skips other keys.
+ * products, the pass splits of lengths 2^min_log..2^max_log (11..28: every
+ * split into the fewest passes of at most 2^10 points, and up to 2^20 one
+ * pass more, each pass of 2^3 points or more) and, when max_log >= 24, the
+ * tile batch (1, 2 or 4) on the longest length, and sets the fastest.  Some
+ * seconds.  save/load keep the result in a small JSON file ({"threshold": T,
+ * "tile_batch": B, "pass_cap": {"<log_len>": cap, ...}, "pass_split":
+ * {"<log_len>": [s0, s1, ...], ...}, "split_us": {"<log_len>": [[[s0, ...],
+ * us], ...]}, "crossover_us": [[n, schoolbook, ntt, karatsuba_lower_bound],
+ * ...]}); the loader reads threshold, tile_batch, pass_cap and pass_split (a
+ * file without tile_batch, pass_cap or pass_split leaves the current value)
+ * and skips other keys.  split_us records the microseconds per product of
+ * every split tried.
  * crossover_us records, per operand size n, microseconds per product of a
  * batch of 256 n x n products on each algorithm and, as a lower bound for one
  * Karatsuba level, three products of ceil(n/2) + 1 limbs on the faster one. */

@@ -13,6 +13,7 @@ import pytest
 
 import _oracle as O
 import cryptography_amd as E
+from cryptography_amd import engine as EN
 
 pytestmark = pytest.mark.gpu
 
@@ -56,6 +57,39 @@ def test_every_transform_length(log_len):
     finally:
         L.ntt_engine_set_threshold(saved)
     assert len(c) == na + nb and len(s) == 2 * na
+    if log_len <= GMP_MAX_LOG:
+        assert np.array_equal(c, O.gmp_mul(a, b))
+        assert np.array_equal(s, O.gmp_mul(a, a))
+    else:
+        _check_mod(a, b, c)
+        _check_mod(a, a, s)
+
+
+@pytest.mark.parametrize("split", [
+    (7, 6), (3, 10), (10, 3), (8, 6), (9, 6), (6, 9), (10, 6), (6, 10), (9, 8), (10, 9), (8, 6, 6), (3, 3, 10),
+    (9, 6, 6), (6, 6, 9), (6, 9, 6), (9, 7, 6), (6, 8, 9), (9, 9, 6), (6, 9, 9), (9, 7, 9), (9, 8, 9),
+    (10, 10, 5), (5, 10, 10), (3, 9, 3, 9), (9, 10, 7),
+])
+def test_explicit_pass_splits(split):
+    """Uneven pass splits (ntt_engine_set_pass_split, chosen by the autotuner):
+    first passes of 2^3..2^10 points (radix-8 rounds or not, with and without
+    cross-twiddle tables for the inner passes), short last passes; product and
+    square exact against GMP (modulo three primes above 2^23), the
+    plan restored afterwards."""
+    log_len = sum(split)
+    na, nb = _sizes(log_len)
+    a = O.splitmix(5000 + log_len, na)
+    b = O.splitmix(6000 + log_len, nb)
+    E.init()
+    before = EN.get_plan(log_len)
+    EN.set_pass_split(log_len, split)
+    try:
+        assert EN.get_plan(log_len) == split
+        c = E.mul(a, b)
+        s = E.sqr(a)
+    finally:
+        EN.set_pass_split(log_len, ())
+    assert EN.get_plan(log_len) == before
     if log_len <= GMP_MAX_LOG:
         assert np.array_equal(c, O.gmp_mul(a, b))
         assert np.array_equal(s, O.gmp_mul(a, a))

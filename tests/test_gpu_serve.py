@@ -3,6 +3,7 @@ result equals the same product computed alone (and GMP), whatever mix it was
 served in, under budgets that make requests wait for admission, with
 submitters on several threads, and with a pinned operand whose resident
 spectrum replaces its transform."""
+import json
 import threading
 
 import numpy as np
@@ -210,15 +211,32 @@ def test_tree_modes_agree():
     assert np.array_equal(P1, Pg) and np.array_equal(B1, Bg)
 
 
+def _clear_splits(lo, hi):
+    for l in range(lo, hi + 1):
+        EN.set_pass_split(l, ())
+
+
 def test_autotune_keeps_results_exact(tmp_path):
-    thr, caps = EN.autotune(11, 21, save=str(tmp_path / "tune.json"))
-    assert 0 <= thr <= 128 and all(8 <= c <= 10 for c in caps.values())
-    for n in (3, 50, 200, 5000, 40000):
-        a = O.splitmix(n, n)
-        b = O.splitmix(n + 1, n + 3)
-        assert np.array_equal(E.mul(a, b), O.gmp_mul(a, b))
-    thr2, caps2 = EN.autotune(11, 21, load=str(tmp_path / "tune.json"))
-    assert thr2 == thr and caps2 == caps
+    """The tuner's crossover and pass splits (every split it tried for 2^11 ..
+    2^21 is a valid plan: parts of 2^3..2^10 points summing to the length);
+    products stay exact under them; save/load give the same settings."""
+    try:
+        thr, plans = EN.autotune(11, 21, save=str(tmp_path / "tune.json"))
+        assert 0 <= thr <= 128
+        for l, p in plans.items():
+            assert sum(p) == l and all(3 <= s <= 10 for s in p), (l, p)
+        for n in (3, 50, 200, 5000, 40000):
+            a = O.splitmix(n, n)
+            b = O.splitmix(n + 1, n + 3)
+            assert np.array_equal(E.mul(a, b), O.gmp_mul(a, b))
+        doc = json.loads((tmp_path / "tune.json").read_text())
+        assert {int(k): tuple(v) for k, v in doc["pass_split"].items()} == plans
+        assert set(doc["split_us"]) == {str(l) for l in plans}
+        _clear_splits(11, 21)
+        thr2, plans2 = EN.autotune(11, 21, load=str(tmp_path / "tune.json"))
+        assert thr2 == thr and plans2 == plans
+    finally:
+        _clear_splits(11, 21)
 
 
 def test_autotune_tile_batch(tmp_path):
@@ -237,3 +255,4 @@ def test_autotune_tile_batch(tmp_path):
         assert EN.get_tile_batch() == tb
     finally:
         EN.set_tile_batch(saved)
+        _clear_splits(24, 24)
