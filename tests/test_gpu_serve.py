@@ -171,6 +171,45 @@ def test_pinned_operand_reuses_its_spectrum():
         pool.close()
 
 
+def test_pinned_spectrum_follows_the_pass_plan():
+    """A resident spectrum is the pinned operand's rows after all forward passes
+    but the last, so it depends on the length's pass plan.  After the plan of
+    that length changes (a new split), the server makes a new spectrum instead
+    of reading the old one under the new plan: every product stays exact."""
+    pool = EN.DevicePool(8_000_000)
+    log_len = 21
+    before = EN.get_plan(log_len)
+    try:
+        p = O.random_operand(7, 1 << 24)        # 2^24-bit operands: L = 2^21
+        op = pool.take(len(p))
+        pool.upload(op, p)
+        host = []
+        for i in range(6):
+            a = O.random_operand(700 + i, (1 << 24) - 1000 * i)
+            oa = pool.take(len(a))
+            pool.upload(oa, a)
+            oc = pool.take(len(a) + len(p))
+            host.append((a, oa, oc))
+            assert E.log_len(len(a), len(p)) == log_len
+        with EN.Server(2 << 30, max_inflight=2) as srv:
+            srv.pin(pool.ptr(op), len(p))
+            for k, split in enumerate(((7, 7, 7), (9, 6, 6), (6, 6, 9))):
+                EN.set_pass_split(log_len, split)
+                for (a, oa, oc) in host[2 * k: 2 * k + 2]:
+                    d = EN.make_descs([(pool.ptr(oa), len(a), pool.ptr(op), len(p), pool.ptr(oc),
+                                        len(a) + len(p), None, 0)])[0]
+                    assert srv.wait(srv.submit(d))[0] == 0
+                assert srv.stats()["spectra_made"] == k + 1
+            srv.unpin(pool.ptr(op), len(p))
+        for (a, oa, oc) in host:
+            assert np.array_equal(pool.download(oc, len(a) + len(p)), O.gmp_mul(a, p))
+    finally:
+        EN.set_pass_split(log_len, ())
+        if EN.get_plan(log_len) != before:
+            EN.set_pass_split(log_len, before)
+        pool.close()
+
+
 def test_close_finishes_queued_requests():
     pool = EN.DevicePool(10_000_000)
     try:
