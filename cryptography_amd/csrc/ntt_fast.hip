@@ -515,47 +515,62 @@ __device__ __forceinline__ void dif_above_last(uint64_t *lds, const TwPair *twf,
     }
 }
 
-/* Thread tid's group in the smallest DIF round of a radix-8 transform
- * (LOGS a multiple of 3: one 8-point group per thread): rows m0 .. m0+7 of
- * tile column c, at tile words q0 + i*C; the same rows go to gb + i*gs in
- * the row. */
+/* The smallest DIF round of an S = 2^LOGS transform (LOGS >= 1) works on
+ * groups of 2^LAST_RL<LOGS> consecutive rows, 8 >> LAST_RL of them per thread. */
 template <int LOGS>
-__device__ __forceinline__ uint32_t last_group(const PassArgs &pa, const FGeom &g, uint32_t tt, uint32_t &c,
-                                               uint32_t &m0, size_t &gb, size_t &gs)
+constexpr int last_rl() { return ((LOGS - 1) % 3) + 1; }
+
+/* Thread tid's group n of that round: rows m0 .. m0 + 2^RL - 1 of tile column
+ * c, at tile words q0 + i*C; the same rows go to gb + i*gs in the row. */
+template <int LOGS>
+__device__ __forceinline__ uint32_t last_group(const PassArgs &pa, const FGeom &g, uint32_t tt, uint32_t tid, int n,
+                                               uint32_t &c, uint32_t &m0, size_t &gb, size_t &gs)
 {
-    uint32_t tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-    c = tid & ((1u << g.logC) - 1);
-    const uint32_t u = tid >> g.logC;
-    m0 = (u & ((1u << (LOGS - 3)) - 1)) << 3;
-    const uint32_t q0 = ((((u >> (LOGS - 3)) << LOGS) + m0) << g.logC) + c;
+    constexpr int RL = last_rl<LOGS>();
+    const uint32_t gid = tid + (uint32_t)n * FTPB;
+    c = gid & ((1u << g.logC) - 1);
+    const uint32_t u = gid >> g.logC;
+    m0 = (u & ((1u << (LOGS - RL)) - 1)) << RL;
+    const uint32_t q0 = ((((u >> (LOGS - RL)) << LOGS) + m0) << g.logC) + c;
     uint32_t m, r;
     gb = res_at(ft_addr(LOGS, g, tt, q0, m, r), pa.pad_log);
     gs = res_at(ft_addr(LOGS, g, tt, q0 + (1u << g.logC), m, r), pa.pad_log) - gb;
     return q0;
 }
 
-/* The smallest DIF round with the cross-twiddle chain (A, B looked up at
- * tile start), stored from registers: no LDS round trip and no block
- * barrier before the tile leaves (measured on the first pass of a 2^30-bit
- * product: 3.89 -> 3.43 ms, and 78 -> 70 VGPRs). */
+/* The smallest DIF round with the cross-twiddle chain (for an 8-point group
+ * and one group per thread, A and B looked up at tile start; otherwise per
+ * group), stored from registers: no LDS round trip and no block barrier
+ * before the tile leaves (measured on the first pass of a 2^30-bit product:
+ * 3.89 -> 3.43 ms, and 78 -> 70 VGPRs). */
 template <int PI, int LOGS>
 __device__ __forceinline__ void dif_last_store(const PassArgs &pa, const FGeom &g, uint32_t tt, const uint64_t *lds,
                                                uint64_t *x, const XtwCtx &xc)
 {
-    uint32_t c, m0;
-    size_t gb, gs;
-    const uint32_t q0 = last_group<LOGS>(pa, g, tt, c, m0, gb, gs);
-    uint64_t v[8];
+    constexpr int RL = last_rl<LOGS>(), R = 1 << RL, NG = 8 >> RL;
+    uint32_t tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
 #pragma unroll
-    for (int i = 0; i < 8; i++) v[i] = lds[LP(q0 + ((uint32_t)i << g.logC))];
-    const_dif<PI, 3, 0>(v);
-    chain_apply<PI, 3>(v, xc.ta, xc.tb);
+    for (int n = 0; n < NG; n++) {
+        uint32_t c, m0;
+        size_t gb, gs;
+        const uint32_t q0 = last_group<LOGS>(pa, g, tt, tid, n, c, m0, gb, gs);
+        uint64_t v[R];
 #pragma unroll
-    for (int i = 0; i < 8; i++) x[gb + i * gs] = v[i];
+        for (int i = 0; i < R; i++) v[i] = lds[LP(q0 + ((uint32_t)i << g.logC))];
+        const_dif<PI, RL, 0>(v);
+        if constexpr (RL == 3 && xt_ahead<LOGS>()) {
+            chain_apply<PI, 3>(v, xc.ta, xc.tb);
+        } else {
+            const uint32_t r = xc.col0 + c;
+            chain_twiddle<PI, RL>(v, xc.tw, (r * brev_n(m0, LOGS)) << xc.shift, (r << (LOGS - RL)) << xc.shift);
+        }
+#pragma unroll
+        for (int i = 0; i < R; i++) x[gb + i * gs] = v[i];
+    }
 }
 
-/* The top DIT round of a C * S = 4096 tile (radix 8, LOGS a multiple of 3),
+/* The top DIT round of a C * S = 4096 tile (radix 8 for every S >= 8),
  * stored from registers: thread tid's group is tile words tid + k*FTPB,
  * which tile_span places at gb + k*gs. */
 template <int PI, int LOGS>
@@ -784,43 +799,66 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
     if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false>(xc, g.logC);
     if constexpr (!FIRST) WAIT_VMCNT(0);
     lds_sync();
-    if constexpr (XTAB && !FIRST && LOGS % 3 == 0 && LOGS >= 6) {
+    if constexpr (XTAB && !FIRST && LOGS >= 3) {
         /* The smallest round stores from registers, and the next tile's
-         * transfer starts once every thread has read its group, so it is in
+         * transfer starts once every thread has read its groups, so it is in
          * flight during that round, the table products and the stores
          * (3.51 -> 3.43 ms at L = 2^27, two tiles per block).  The table
-         * entries of a group are rows m0 .. m0+7 of its column. */
+         * entries of a group are rows m0 .. m0 + 2^RL - 1 of its column. */
+        constexpr int RL = last_rl<LOGS>(), R = 1 << RL, NG = 8 >> RL;
         const uint64_t *xt = pa.xtab + ((size_t)PI << (LOGS + g.logT));
         for (;;) {
             dif_above_last<PI, LOGS, LOGS - 1>(lds, twf, g.logC);
-            uint32_t c, m0;
-            size_t gb, gs;
-            const uint32_t q0 = last_group<LOGS>(pa, g, tt, c, m0, gb, gs);
+            uint32_t tid = threadIdx.x;
+            asm volatile("" : "+v"(tid));
+            /* the groups and their table entries are read before the next
+             * tile's transfer is issued */
             uint64_t v[8], w[8];
+            size_t gb0 = 0, gs0 = 0;
 #pragma unroll
-            for (int i = 0; i < 8; i++) v[i] = lds[LP(q0 + ((uint32_t)i << g.logC))];
+            for (int n = 0; n < NG; n++) {
+                uint32_t c, m0;
+                size_t gb, gs;
+                const uint32_t q0 = last_group<LOGS>(pa, g, tt, tid, n, c, m0, gb, gs);
+                if (n == 0) {
+                    gb0 = gb;
+                    gs0 = gs;
+                }
 #pragma unroll
-            for (int i = 0; i < 8; i++) w[i] = xt[((size_t)(m0 + i) << g.logT) + xc.col0 + c];
+                for (int i = 0; i < R; i++) v[n * R + i] = lds[LP(q0 + ((uint32_t)i << g.logC))];
+#pragma unroll
+                for (int i = 0; i < R; i++) w[n * R + i] = xt[((size_t)(m0 + i) << g.logT) + xc.col0 + c];
+            }
+            const uint32_t tcur = tt;
             const bool more = t + 1 < tend;
             if (more) {
                 tt = xtab_tile(++t, colTiles, pa.ntiles);
                 lds_sync();       /* every group read: the tile buffer is free */
                 tile_dma<LOGS>(pa, x, g, tt, lds);
             }
-            /* the last stage in [0, 4p): the Montgomery product by a table
-             * entry below p returns (0, 2p) */
-            const_dif<PI, 3, 0, true>(v);
 #pragma unroll
-            for (int i = 0; i < 8; i++) x[gb + i * gs] = mont_mul_lazy<PI>(v[i], w[i]);
+            for (int n = 0; n < NG; n++) {
+                size_t gb = gb0, gs = gs0;
+                if (n > 0) {
+                    uint32_t c, m0;
+                    (void)last_group<LOGS>(pa, g, tcur, tid, n, c, m0, gb, gs);
+                }
+                /* the last stage in [0, 4p): the Montgomery product by a
+                 * table entry below p returns (0, 2p) */
+                const_dif<PI, RL, 0, true>(v + n * R);
+#pragma unroll
+                for (int i = 0; i < R; i++) x[gb + i * gs] = mont_mul_lazy<PI>(v[n * R + i], w[n * R + i]);
+            }
             if (!more) return;
             xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
             WAIT_VMCNT(0);
             lds_sync();
         }
     }
-    /* the first pass of a radix-8 transform stores its smallest round from
-     * registers */
-    constexpr bool REGSTORE = FIRST && LOGS % 3 == 0 && LOGS >= 6 && xt_ahead<LOGS>();
+    /* the loop below serves the first pass and the chain passes (the table
+     * passes, S >= 8, returned above); the first pass stores its smallest
+     * round from registers */
+    constexpr bool REGSTORE = FIRST && LOGS >= 4;
     for (;;) {
         uint32_t tid;
         if constexpr (FIRST) {
@@ -841,35 +879,19 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
                 dif_all<PI, LOGS, LOGS - 4, true>(lds, twf, g.logC, xc);
             }
         } else {
-            /* chain: cross twiddles in the last round; table: the last stage
-             * stays in [0, 4p), as the store's Montgomery product by a table
-             * entry below p returns (0, 2p) */
-            dif_all<PI, LOGS, LOGS - 1, !XTAB, XTAB>(lds, twf, g.logC, xc);
+            /* cross twiddles by chain in the last round */
+            dif_all<PI, LOGS, LOGS - 1, true>(lds, twf, g.logC, xc);
         }
         tid = threadIdx.x;
         asm volatile("" : "+v"(tid));
         if constexpr (!REGSTORE) {
             size_t base, step;
             tile_span<LOGS>(pa, g, tt, tid, base, step);
-            if constexpr (XTAB) {
-                const uint64_t *xt = pa.xtab + ((size_t)PI << (LOGS + g.logT));
-                uint64_t w[FT_PER];
 #pragma unroll
-                for (int k = 0; k < FT_PER; k++) {
-                    const uint32_t q = tid + k * FTPB;
-                    const uint32_t m = (q >> g.logC) & ((1u << LOGS) - 1), c = q & ((1u << g.logC) - 1);
-                    w[k] = xt[((size_t)m << g.logT) + xc.col0 + c];
-                }
-#pragma unroll
-                for (int k = 0; k < FT_PER; k++)
-                    x[base + k * step] = mont_mul_lazy<PI>(lds[LP(tid + k * FTPB)], w[k]);
-            } else {
-#pragma unroll
-                for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
-            }
+            for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
         }
         if (++t >= tend) break;
-        tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;
+        tt = t;
         lds_sync();
         if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, tt, lds);
         xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
@@ -935,10 +957,9 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
         /* otherwise the inverse cross twiddles go into the first round; the
          * table's Montgomery products (an entry below p) leave the tile
          * below 2p */
-        if constexpr (LOADCHAIN && LOGS % 3 == 0 && LOGS >= 6) {
-            /* radix-8 rounds: the top round stores from registers (the
-             * inverse passes of a 2^30-bit product: 1.89 -> 1.83 ms and
-             * 2.14 -> 2.11 ms) */
+        if constexpr (LOADCHAIN) {
+            /* the top round stores from registers (the inverse passes of a
+             * 2^30-bit product: 1.89 -> 1.83 ms and 2.14 -> 2.11 ms) */
             dit_all<PI, LOGS, LOGS - 4, false, XTAB>(lds, twi, g.logC, xc);
             tid = threadIdx.x;
             asm volatile("" : "+v"(tid));
