@@ -248,7 +248,7 @@ int ntt_engine_autotune_load(const char *path)
                         if (!in.str(lk_, sizeof lk_) || !in.eat(':') || !in.eat('[')) { in.ok = false; break; }
                         const int l = atoi(lk_);
                         if (l < 1 || l > NTT_MAX_LOG_LEN) return fail(NTT_ERR_ARG, "%s: pass split for 2^%d", path, l);
-                        int np = 0, sum = 0;
+                        int np = 0, sum = 0, lo = FT_MAX_LOGS;
                         do {
                             if (!in.num(v)) break;
                             const int ls = (int)v;
@@ -256,10 +256,13 @@ int ntt_engine_autotune_load(const char *path)
                                 return fail(NTT_ERR_ARG, "%s: pass split for 2^%d out of range", path, l);
                             sp[l][np++] = (uint8_t)ls;
                             sum += ls;
+                            lo = std::min(lo, ls);
                         } while (in.ok && in.eat(','));
                         if (!in.ok || !in.eat(']')) { in.ok = false; break; }
                         if (np == 0 || sum != l)
                             return fail(NTT_ERR_ARG, "%s: pass split for 2^%d sums to 2^%d", path, l, sum);
+                        if (np > 1 && lo < 3)       /* passes of 2^3..2^10 points, as ntt_engine_set_pass_split */
+                            return fail(NTT_ERR_ARG, "%s: pass split for 2^%d has a pass of 2^%d points", path, l, lo);
                         sn[l] = (uint8_t)np;
                     } while (in.ok && in.eat(','));
                     if (in.ok && !in.eat('}')) in.ok = false;

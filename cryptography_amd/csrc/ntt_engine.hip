@@ -1101,7 +1101,8 @@ int ntt_engine_set_pass_split(int log_len, const int *log_s, int npass)
     if (npass < 1 || npass > 8 || !log_s) return fail(NTT_ERR_ARG, "pass split for 2^%d: %d passes", log_len, npass);
     int sum = 0;
     for (int i = 0; i < npass; i++) {
-        if (log_s[i] < 1 || log_s[i] > FT_MAX_LOGS)
+        /* passes of 2^3..2^10 points (a single pass may be shorter) */
+        if (log_s[i] < (npass == 1 ? 1 : 3) || log_s[i] > FT_MAX_LOGS)
             return fail(NTT_ERR_ARG, "pass split for 2^%d: a pass of 2^%d points", log_len, log_s[i]);
         sum += log_s[i];
     }

@@ -118,8 +118,8 @@ void ntt_engine_set_threshold(int limbs);
 int ntt_engine_set_pass_cap(int log_len, int cap);
 int ntt_engine_get_pass_cap(int log_len);
 /* An explicit pass split for transforms of length 2^log_len: npass passes
- * of 2^log_s[0], 2^log_s[1], ... points (first pass first; each 1..10, the
- * sum log_len, npass <= 8).  It takes precedence over the pass cap; npass = 0
+ * of 2^log_s[0], 2^log_s[1], ... points (first pass first; each 3..10, or one
+ * pass of log_len; the sum log_len, npass <= 8).  It takes precedence over the pass cap; npass = 0
  * clears it.  Set by the autotuner.  A change applies to later calls: batch
  * graphs and product trees keep the plan they were captured with, and a
  * server rebuilds a resident spectrum made under another plan. */

@@ -1,8 +1,9 @@
-"""The tuning-file reader (ntt_engine_autotune_load) on the CPU: it only sets
-the engine's host-side parameters, so no GPU is needed.  Valid files set the
-threshold, tile batch and pass caps; extra keys (the recorded crossover
-samples) are skipped; malformed or out-of-range files are rejected and change
-nothing.  Also the committed tuning file parses."""
+"""The tuning-file reader (ntt_engine_autotune_load) and the pass-plan calls on
+the CPU: they only set the engine's host-side parameters, so no GPU is needed.
+Valid files set the threshold, tile batch, pass caps and pass splits; extra
+keys (the recorded crossover and split samples) are skipped; malformed or
+out-of-range files are rejected and change nothing.  The committed tuning file
+parses and its splits are the fastest it recorded."""
 import ctypes
 import json
 import os
@@ -126,6 +127,7 @@ def test_pass_split_file_sets_plans(restore, tmp_path):
 @pytest.mark.parametrize("split", [
     '{"21": [6, 6, 8]}', '{"21": [11, 10]}', '{"21": [6, 6, 9, 0]}', '{"31": [10, 10, 11]}', '{"21": []}',
     '{"21": [6.5, 5.5, 9]}', '{"21": 21}', '{"2": [1, 1, 0]}', '{"24": [3, 3, 3, 3, 3, 3, 3, 1, 2]}',
+    '{"12": [2, 10]}', '{"12": [10, 2]}', '{"13": [1, 6, 6]}',
 ])
 def test_bad_pass_splits_are_rejected_and_change_nothing(restore, tmp_path, split):
     L = restore
@@ -138,7 +140,7 @@ def test_set_pass_split_checks_and_cap_clears_it(restore):
     L = restore
     EN.set_pass_split(24, (9, 9, 6))
     assert EN.get_plan(24) == (9, 9, 6)
-    for bad in ((9, 9, 5), (11, 13), (0, 24), tuple([3] * 9)):
+    for bad in ((9, 9, 5), (11, 13), (0, 24), tuple([3] * 9), (2, 10, 10, 2), (10, 10, 2, 2)):
         with pytest.raises(EN.NttError):
             EN.set_pass_split(24, bad)
         assert EN.get_plan(24) == (9, 9, 6)
@@ -147,5 +149,8 @@ def test_set_pass_split_checks_and_cap_clears_it(restore):
     assert EN.get_plan(24) == _plan_by_cap(24, 9)
     EN.set_pass_split(24, (6, 9, 9))
     EN.set_pass_split(24, ())
+    EN.set_pass_split(2, (2,))            # one pass may be shorter than 2^3
+    assert EN.get_plan(2) == (2,)
+    EN.set_pass_split(2, ())
     assert EN.get_plan(24) == _plan_by_cap(24, 9)
     assert L.ntt_engine_set_pass_cap(24, 0) == 0
