@@ -1,11 +1,13 @@
 /*
  * modarith.h — exact 64-bit modular arithmetic for the three NTT primes.
  *
- * Host and device.  The primes are p = c*2^k + 1 < 2^62 with k >= 41, so
- * lazily reduced values in [0, 4p) still fit one u64 word (Harvey's lazy
- * butterflies) and every power-of-two transform length up to 2^41 has roots.
- * Arithmetic follows the reference's primitives (ntt/lib/primes.h:193-279,
- * ntt/lib/shoup.h:45-112) with the build's primes and lazy ranges.
+ * Host and device.  The primes are p = 2^60 - c*2^32 + 1 (c < 2^8), so
+ * p = 1 (mod 2^32), every power-of-two transform length up to 2^35 has
+ * roots, lazily reduced values below 16p still fit one u64 word, and one
+ * subtraction of floor(x / 2^60) * p brings any 64-bit x below 2p
+ * (reduce_any).  Arithmetic follows the reference's primitives
+ * (ntt/lib/primes.h:193-279, ntt/lib/shoup.h:45-112) with the build's primes
+ * and lazy ranges.
  */
 #ifndef CRYPTOGRAPHY_AMD_MODARITH_H
 #define CRYPTOGRAPHY_AMD_MODARITH_H
@@ -23,13 +25,15 @@
 
 namespace nttc {
 
-/* prime i: value, exponent k (p = c*2^k+1), generator of Z_p^* */
+/* prime i: value (2^60 - c*2^32 + 1 for c = 96, 120, 168), exponent k of
+ * the largest power of two dividing p - 1 (p = c'*2^k + 1, c' odd),
+ * generator of Z_p^* */
 __host__ __device__ constexpr uint64_t prime(int i)
 {
-    return i == 0 ? 0x3fffc00000000001ull : i == 1 ? 0x3fffbe0000000001ull : 0x3fff840000000001ull;
+    return i == 0 ? 0x0fffffa000000001ull : i == 1 ? 0x0fffff8800000001ull : 0x0fffff5800000001ull;
 }
-__host__ __device__ constexpr uint64_t prime_gen(int i) { return i == 0 ? 11 : i == 1 ? 3 : 19; }
-__host__ __device__ constexpr int prime_k(int i) { return i == 0 ? 46 : i == 1 ? 41 : 42; }
+__host__ __device__ constexpr uint64_t prime_gen(int i) { return i == 0 ? 3 : i == 1 ? 3 : 6; }
+__host__ __device__ constexpr int prime_k(int i) { return i == 0 ? 37 : i == 1 ? 35 : 35; }
 
 /* -p^-1 mod 2^64 by Newton iteration (each step doubles the correct bits). */
 __host__ __device__ constexpr uint64_t neg_inv64(uint64_t p)
@@ -97,6 +101,24 @@ __host__ __device__ __forceinline__ uint64_t reduce2(uint64_t x)
     return x >= Prime<PI>::P2 ? x - Prime<PI>::P2 : x;
 }
 
+/* any x < 2^64 -> [0, 2p), congruent: x - q*p for q = floor(x / 2^60) < 16.
+ * With x = q*2^60 + s (s < 2^60) and d = 2^60 - p = c*2^32 - 1, the result is
+ * s + q*d < 2^60 + 15d < 2p (as 17d < 2^60), and it is >= 0 as q*p <= x.
+ * p = ph*2^32 + 1, so q*p = q + (q*ph)*2^32 with q*ph < 2^32: a shift, one
+ * 32-bit multiply and a 64-bit subtraction. */
+template <int PI>
+__host__ __device__ __forceinline__ uint64_t reduce_any(uint64_t x)
+{
+    static_assert(prime(PI) < (1ull << 60) && 17 * ((1ull << 60) - prime(PI)) < (1ull << 60),
+                  "p just below 2^60");
+    constexpr uint32_t PH = (uint32_t)(prime(PI) >> 32);
+    const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32);
+    const uint32_t q = xh >> 28;
+    const uint32_t lo = xl - q;
+    const uint32_t hi = xh - q * PH - (uint32_t)(xl < q);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 /* Shoup product a*w mod p for any a < 2^64, w < p, wp = floor(w*2^64/p);
  * result in [0, 2p).  q = hi(a*wp) is floor(a*w/p) or one less. */
 template <int PI>
@@ -130,6 +152,27 @@ __host__ __device__ __forceinline__ bool crt_below_p0(uint64_t x0, uint64_t x1, 
     const uint64_t m1 = x0 >= prime(1) ? x0 - prime(1) : x0;
     const uint64_t m2 = x0 >= prime(2) ? x0 - prime(2) : x0;
     return (m1 == x1) & (m2 == x2);
+}
+
+/* The integer X < 2 p0 with residues x_i in [0, p_i), if there is one: X is
+ * x0 (Garner digits a1 = a2 = 0, crt_below_p0) or x0 + p0 (a1 = 1, a2 = 0:
+ * x0 + p0 = x1 mod p1 and = x2 mod p2).  p0 > p1, p2 and p0 - p_i < p_i, so
+ * (x0 + p0) mod p_i = (x0 mod p_i) + (p0 - p_i), less p_i when it reaches
+ * p_i.  Sets ok to false when neither holds. */
+__host__ __device__ __forceinline__ uint64_t crt_below_2p0(uint64_t x0, uint64_t x1, uint64_t x2, bool &ok)
+{
+    static_assert(prime(0) > prime(1) && prime(0) > prime(2) && prime(0) < 2 * prime(1) &&
+                      prime(0) < 2 * prime(2),
+                  "p1, p2 < p0 < 2 p1, 2 p2");
+    constexpr uint64_t D1 = prime(0) - prime(1), D2 = prime(0) - prime(2);
+    const uint64_t m1 = x0 >= prime(1) ? x0 - prime(1) : x0;
+    const uint64_t m2 = x0 >= prime(2) ? x0 - prime(2) : x0;
+    uint64_t n1 = m1 + D1, n2 = m2 + D2;
+    n1 = n1 >= prime(1) ? n1 - prime(1) : n1;
+    n2 = n2 >= prime(2) ? n2 - prime(2) : n2;
+    const bool lo = (m1 == x1) & (m2 == x2), hi = (n1 == x1) & (n2 == x2);
+    ok = lo | hi;
+    return hi && !lo ? x0 + prime(0) : x0;
 }
 
 /* Montgomery product a*b*2^-64 mod p for a*b < 2^64 * p; result in [0, p). */
@@ -181,8 +224,9 @@ __host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
     y = reduce2<PI>(d);
 }
 
-/* Montgomery product a*b*2^-64 mod p for a*b < 8p^2 (a < 4p and b < 2p);
- * result in (0, 3p), and in (0, 2p) when a, b < 2p or when a < 4p, b < p.
+/* Montgomery product a*b*2^-64 mod p for any a < 2^64 and b < 2p; result in
+ * (T/2^64, T/2^64 + p] for T = a*b (derivation below): below 3p always,
+ * below 2p when T < 2^64 p (b < p, or a < 2^63), below p + 1 when T < 2^64.
  * Every prime here is 1 mod 2^32 (p = ph*2^32 + 1), so p^-1 = 1 (mod 2^32)
  * and the reduction is two subtractive 32-bit steps with free multipliers:
  * T = a*b = hi*2^64 + l1*2^32 + l0; T - l0*p = 2^32 (hi*2^32 + D) with
@@ -190,7 +234,8 @@ __host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
  * 2^32 (hi + floor(D / 2^32) - s0*ph).  That last value is
  * (T - m p) / 2^64 for m = l0 + s0*2^32 < 2^64, so it lies in
  * (T/2^64 - p, T/2^64]; adding p gives (T/2^64, T/2^64 + p], and
- * T/2^64 < 8p^2/2^64 < 2p as p < 2^62 (below p when T < 4p^2).  The 128-bit product takes four
+ * T/2^64 < 2p for T < 2^65 p.  floor(D / 2^32) lies in [-ph, 0], an int32
+ * as ph < 2^28.  The 128-bit product takes four
  * 32x32->64 multiplies and each step one more: no carries to compare. */
 template <int PI>
 __host__ __device__ __forceinline__ uint64_t mont_mul_lazy(uint64_t a, uint64_t b)
@@ -229,8 +274,8 @@ __host__ __device__ __forceinline__ void dif_bfly(uint64_t &x, uint64_t &y, uint
 }
 
 /* Inverse (decimation-in-time) butterfly in Harvey's [0,4p) form:
- * (x, y) -> (x + y*w, x - y*w) for x, y in [0, 4p), both outputs in [0, 4p)
- * (4p < 2^64 for these primes).  Only x is reduced, to [0, 2p); y*w comes
+ * (x, y) -> (x + y*w, x - y*w) for x, y in [0, 4p), both outputs in [0, 4p).
+ * Only x is reduced, to [0, 2p); y*w comes
  * from Shoup in [0, 2p) for any y.  Inverse passes therefore leave residues
  * in [0, 4p); whatever reads them last reduces twice. */
 template <int PI>

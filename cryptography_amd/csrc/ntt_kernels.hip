@@ -87,18 +87,22 @@ __global__ void __launch_bounds__(256) k_twiddle_gen(TwPair *tw, uint64_t r2_0, 
 /* ------------------------------------------------------------------------ */
 
 /* x = a0 + a1*p0 + a2*p0*p1 in the reference's mixed-radix form
- * (garner.hip:135-173); the product coefficients are < L*(2^16-1)^2 < 2^62
- * < p0, so a1 = a2 = 0 for an intact product and the value is a0 = x0.  Any
- * nonzero a1/a2 is a corrupted product and raises the integrity flag; the
- * test that both are 0 needs no product (crt_below_p0). */
+ * (garner.hip:135-173).  A product coefficient sums at most L/2 products of
+ * 16-bit coefficients, so it is below (L/2)(2^16-1)^2: below p0 (> 2^60 -
+ * 2^45) up to L = 2^29, below 2 p0 at L = 2^30.  An intact product
+ * therefore has a2 = 0 and a1 = 0 or 1: the value is x0 or x0 + p0.  Any
+ * other digits are a corrupted product and raise the integrity flag; the
+ * test needs no product (crt_below_2p0). */
 __device__ __forceinline__ uint64_t garner3(uint64_t r0, uint64_t r1, uint64_t r2, int &bad)
 {
-    /* inverse-pass output in [0, 4p) */
-    const uint64_t x0 = reduce1<0>(reduce2<0>(r0));
-    const uint64_t x1 = reduce1<1>(reduce2<1>(r1));
-    const uint64_t x2 = reduce1<2>(reduce2<2>(r2));
-    bad |= !crt_below_p0(x0, x1, x2);
-    return x0;
+    /* inverse-pass output: any 64-bit value */
+    const uint64_t x0 = reduce1<0>(reduce_any<0>(r0));
+    const uint64_t x1 = reduce1<1>(reduce_any<1>(r1));
+    const uint64_t x2 = reduce1<2>(reduce_any<2>(r2));
+    bool ok;
+    const uint64_t X = crt_below_2p0(x0, x1, x2, ok);
+    bad |= !ok;
+    return X;
 }
 
 /* Inclusive scan of 256 (or 1024) cells in LDS, lower thread index first. */

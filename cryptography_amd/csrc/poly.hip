@@ -6,7 +6,7 @@
  * are reduced mod each prime into the residue rows, the forward passes, the
  * fused pointwise pass and the inverse passes run as for an integer product,
  * and a full three-digit Garner rebuilds every coefficient modulo
- * P = p0 p1 p2 (> 2^185).  f * g mod (X^n + 1) twists the inputs by psi^k
+ * P = p0 p1 p2 (> 2^179).  f * g mod (X^n + 1) twists the inputs by psi^k
  * (psi a primitive 2n-th root of unity) and untwists the result by psi^-k.
  * Coefficients are < 2^64, so |c_k| < n 2^128 <= 2^156 < P / 2: the residue
  * modulo P is exact (a negative negacyclic coefficient c is returned as
@@ -61,12 +61,12 @@ __global__ void __launch_bounds__(256) k_poly_load(uint64_t *res, size_t len, ui
     res[((size_t)op * NTT_NPRIMES + pi) * len + res_at(k, pad_log)] = r;
 }
 
-/* r: inverse-pass output in [0, 4p). */
+/* r: inverse-pass output, any 64-bit value. */
 template <int PI>
 __device__ __forceinline__ uint64_t store_one(uint64_t r, size_t k, int logn, int nega, const TwPair *tw)
 {
     if (nega) r = times_root_pow<PI>(r, tw, (0u - (uint32_t)(k << (29 - logn))) & ((1u << NTT_ROOT_LOG) - 1));
-    return reduce1<PI>(reduce2<PI>(r));
+    return reduce1<PI>(reduce_any<PI>(r));
 }
 
 /* out[3k .. 3k+2] = the value in [0, P) with residues r0, r1, r2: mixed radix

@@ -11,7 +11,7 @@
  *
  * out[3k .. 3k+2] (little-endian words) = coefficient k of f * g mod (X^n - 1)
  * (negacyclic = 0) or mod (X^n + 1) (negacyclic = 1), as its residue modulo
- * P = p0 p1 p2 (P > 2^185): exact for every input (|c_k| < n 2^128 < P/2), a
+ * P = p0 p1 p2 (P > 2^179): exact for every input (|c_k| < n 2^128 < P/2), a
  * negative negacyclic coefficient c reads as P + c.  f and g hold n
  * coefficients < 2^64 each; n is a power of two in [2, 2^NTT_POLY_MAX_LOG].
  * Returns an ntt_status (ntt_engine.h).

@@ -13,14 +13,15 @@
 
 typedef unsigned __int128 u128;
 
-/* Three primes c*2^k+1 < 2^62 (so lazy values in [0,4p) fit a u64 word).
+/* The library's three primes 2^60 - c*2^32 + 1 (c = 96, 120, 168), each
+ * c'*2^k + 1 with c' odd (lazy values below 16p fit a u64 word).
  * Generators were found by testing g^((p-1)/q) != 1 for every prime q | p-1;
  * tests/test_oracle.py (test_primes_form_and_generators) re-proves primality, the factorisation of p-1
  * and the generator property from scratch. */
-static const uint64_t PR[ORC_NPRIMES]  = {0x3fffc00000000001ull, 0x3fffbe0000000001ull,
-                                          0x3fff840000000001ull};
-static const int      PK[ORC_NPRIMES]  = {46, 41, 42};
-static const uint64_t PG[ORC_NPRIMES]  = {11, 3, 19};
+static const uint64_t PR[ORC_NPRIMES]  = {0x0fffffa000000001ull, 0x0fffff8800000001ull,
+                                          0x0fffff5800000001ull};
+static const int      PK[ORC_NPRIMES]  = {37, 35, 35};
+static const uint64_t PG[ORC_NPRIMES]  = {3, 3, 6};
 
 uint64_t orc_prime(int i) { return PR[i]; }
 int orc_prime_k(int i) { return PK[i]; }
@@ -31,7 +32,7 @@ uint64_t orc_prime_gen(int i) { return PG[i]; }
  * plain u128 remainder (the textbook definition). */
 uint64_t orc_addmod(uint64_t a, uint64_t b, uint64_t p)
 {
-    uint64_t s = a + b;             /* a,b < p < 2^62: no wrap */
+    uint64_t s = a + b;             /* a,b < p < 2^60: no wrap */
     return s >= p ? s - p : s;
 }
 uint64_t orc_submod(uint64_t a, uint64_t b, uint64_t p) { return a >= b ? a - b : a + p - b; }

@@ -3,8 +3,9 @@
  * against 128-bit integer arithmetic, for the three NTT primes: the value and
  * the documented output range of the Montgomery product, the Shoup product
  * (and its small-operand form), both butterflies (the inverse one also with a
- * negated twiddle), the compile-time roots, and the CRT test that three
- * residues are those of one integer below p0 (against Garner's digits).
+ * negated twiddle), the one-step reduction of any 64-bit value, the
+ * compile-time roots, and the CRT tests that three residues are those of one
+ * integer below p0, or below 2 p0 (against Garner's digits).
  * Prints "bad N" and exits non-zero on any mismatch.
  */
 #include "../../cryptography_amd/csrc/modarith.h"
@@ -85,6 +86,18 @@ static void check(long iters)
         expect(a < 4 * p && a % p == (X % p + t) % p, "dit neg sum", PI, X, Y);
         expect(b < 4 * p && b % p == (X % p + p - t) % p, "dit neg diff", PI, X, Y);
     };
+    /* any 64-bit value to [0, 2p) in one step */
+    auto red = [&](uint64_t x) {
+        const uint64_t r = reduce_any<PI>(x);
+        expect(r < 2 * p && r % p == x % p, "reduce_any", PI, x, r);
+    };
+    for (uint64_t x : edges) red(x);
+    for (uint64_t q = 0; q < 16; q++)
+        for (uint64_t x : {q << 60, (q << 60) - 1, (q << 60) + 1, q * p, q * p - 1, q * p + 2 * p - 1})
+            red(x);
+    red(~0ull);
+    red(16 * p - 1);
+    for (long i = 0; i < iters; i++) red(rnd());
     for (uint64_t a : edges)
         for (uint64_t b : edges) {
             if (a < 4 * p && b < 2 * p) mont(a, b);
@@ -130,22 +143,39 @@ static void check_crt(long iters)
         x1 = (uint64_t)(X % p1);
         x2 = (uint64_t)(X % p2);
     };
+    /* a1 in {0, 1} and a2 = 0: the integer below 2 p0 */
+    auto digits_small = [&](uint64_t x0, uint64_t x1, uint64_t x2) {
+        const uint64_t a1 = (uint64_t)((u128)((x1 + p1 - x0 % p1) % p1) * i01 % p1);
+        const uint64_t b = (uint64_t)(((u128)x0 + (u128)a1 * p0) % p2);
+        const uint64_t a2 = (uint64_t)((u128)((x2 + p2 - b) % p2) * i012 % p2);
+        return a1 <= 1 && a2 == 0;
+    };
     auto one = [&](u128 X, bool below) {
         uint64_t x0, x1, x2;
         res(X, x0, x1, x2);
         const bool got = crt_below_p0(x0, x1, x2);
         expect(got == digits_zero(x0, x1, x2), "crt vs Garner digits", 0, (uint64_t)(X >> 64), (uint64_t)X);
         expect(got == below, "crt below p0", 0, (uint64_t)(X >> 64), (uint64_t)X);
+        bool ok;
+        const uint64_t v = crt_below_2p0(x0, x1, x2, ok);
+        const bool below2 = X < 2 * (u128)p0;
+        expect(ok == digits_small(x0, x1, x2), "crt2 vs Garner digits", 0, (uint64_t)(X >> 64), (uint64_t)X);
+        expect(ok == below2 && (!ok || v == (uint64_t)X), "crt below 2 p0", 0, (uint64_t)(X >> 64), (uint64_t)X);
     };
-    const u128 edges[] = {0, 1, p2 - 1, p2, p2 + 1, p1 - 1, p1, p1 + 1, p0 - 1, p0, p0 + 1, 2 * (u128)p0,
-                          (u128)p0 * p1, (u128)p0 * p1 - 1, (u128)p0 * p1 + p0 - 1, ~(u128)0};
+    const u128 edges[] = {0, 1, p2 - 1, p2, p2 + 1, p1 - 1, p1, p1 + 1, p0 - 1, p0, p0 + 1,
+                          p0 + p2, p0 + p1, 2 * (u128)p0 - 1, 2 * (u128)p0, 2 * (u128)p0 + 1, (u128)p0 * p1,
+                          (u128)p0 * p1 - 1, (u128)p0 * p1 + p0 - 1, ~(u128)0};
     for (u128 X : edges) one(X, X < p0);
     for (long i = 0; i < iters; i++) {
         one(rnd() % p0, true);
         one(p2 + rnd() % (p0 - p2), true);
+        one(p0 + rnd() % p0, false);
         one(p0 + (((u128)rnd() << 64) | rnd()) % ((~(u128)0) - p0), false);
         const uint64_t x0 = rnd() % p0, x1 = rnd() % p1, x2 = rnd() % p2;
         expect(crt_below_p0(x0, x1, x2) == digits_zero(x0, x1, x2), "crt random tuple", 0, x0, x1);
+        bool ok;
+        (void)crt_below_2p0(x0, x1, x2, ok);
+        expect(ok == digits_small(x0, x1, x2), "crt2 random tuple", 0, x0, x1);
     }
 }
 

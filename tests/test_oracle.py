@@ -58,13 +58,15 @@ def factor_small(n):
 def test_primes_form_and_generators():
     L = O.orc()
     P = O.primes()
-    assert len(set(P)) == 3
+    assert len(set(P)) == 3 and P[0] == max(P) and 2 ** 60 - P[0] < 2 ** 45
     for i, p in enumerate(P):
         k = L.orc_prime_k(i)
-        assert p < 2 ** 62 and is_prime(p)
+        # just below 2^60 and 1 mod 2^32: 2^60 - p < 2^60 / 17 (one-step reduction of
+        # any 64-bit value) and < 2^45 for p0 (products of 2^29 coefficients below p0)
+        assert 17 * (2 ** 60 - p) < 2 ** 60 and p % 2 ** 32 == 1 and is_prime(p)
         c = (p - 1) >> k
         assert c * 2 ** k + 1 == p and c % 2 == 1 and k >= 32
-        # generator: g^((p-1)/q) != 1 for every prime q | p-1 (p-1 = c * 2^k, c < 2^20)
+        # generator: g^((p-1)/q) != 1 for every prime q | p-1 (p-1 = c * 2^k, c < 2^28)
         g = L.orc_prime_gen(i)
         qs = set(factor_small(c)) | {2}
         assert all(is_prime(q) for q in qs)
