@@ -114,8 +114,14 @@ __host__ __device__ __forceinline__ uint64_t reduce_any(uint64_t x)
     constexpr uint32_t PH = (uint32_t)(prime(PI) >> 32);
     const uint32_t xl = (uint32_t)x, xh = (uint32_t)(x >> 32);
     const uint32_t q = xh >> 28;
+    uint32_t t = q * PH;
+#if defined(__HIP_DEVICE_COMPILE__)
+    /* a plain multiply: merged into xh - q*PH as one 64-bit multiply-add, xh
+     * would first need a move into an even register pair */
+    asm("" : "+v"(t));
+#endif
     const uint32_t lo = xl - q;
-    const uint32_t hi = xh - q * PH - (uint32_t)(xl < q);
+    const uint32_t hi = xh - t - (uint32_t)(xl < q);
     return ((uint64_t)hi << 32) | lo;
 }
 

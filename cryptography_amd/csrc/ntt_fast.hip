@@ -45,6 +45,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "lazy_group.h"
 #include "modarith.h"
 #include "ntt_kernels.h"
 
@@ -157,70 +158,17 @@ __device__ __forceinline__ void tile_span(const PassArgs &pa, const FGeom &g, ui
     step = res_at(ft_addr(LOGS, g, t, tid + FTPB, m, r), pa.pad_log) - base;
 }
 
-/* ---- the smallest stages with compile-time twiddles ------------------------
- * A round whose groups are 2^RL consecutive points (group spacing d = 1) uses
- * only w_(2hs)^e with hs <= 2^(RL-1): fixed powers of fixed roots.  They are
- * immediates here, and e = 0 (a quarter to all of a stage's butterflies)
- * skips the multiply.  Stage s of the round has half-size hs = 2^(RL-1-s),
- * root order 2^(RL-s), pair distance 2^(RL-1-s) in the group. */
-/* LAZY, forward (last stage): the twiddle-1 butterflies leave x + y and
- * x - y + 2p unreduced, in [0, 4p).  LAZY, inverse (first stage): the inputs
- * are below 2p already, so the twiddle-1 butterflies skip their reductions. */
-template <int PI, int RL, int S, int I, bool INV, bool LAZY = false>
-__device__ __forceinline__ void const_stage(uint64_t *x)
-{
-    constexpr int R = 1 << RL;
-    if constexpr (I < R) {
-        constexpr int half = R >> (S + 1);
-        if constexpr ((I & half) == 0) {
-            constexpr int e = I & (half - 1);
-            constexpr int lo = RL - S;
-            if constexpr (e == 0) {
-                if constexpr (INV && LAZY) {
-                    const uint64_t a = x[I], b = x[I + half];
-                    x[I] = a + b;
-                    x[I + half] = a - b + Prime<PI>::P2;
-                } else if constexpr (INV) {
-                    dit_bfly_one<PI>(x[I], x[I + half]);
-                } else if constexpr (LAZY) {
-                    const uint64_t a = x[I], b = x[I + half];
-                    x[I] = a + b;
-                    x[I + half] = a - b + Prime<PI>::P2;
-                } else {
-                    bfly_one<PI>(x[I], x[I + half]);
-                }
-            } else if constexpr (INV) {
-                using T = ConstTw<PI, lo, (1 << lo) - e>;
-                dit_bfly<PI>(x[I], x[I + half], T::w, T::wp);
-            } else {
-                using T = ConstTw<PI, lo, e>;
-                dif_bfly<PI>(x[I], x[I + half], T::w, T::wp);
-            }
-        }
-        const_stage<PI, RL, S, I + 1, INV, LAZY>(x);
-    }
-}
-
-/* DIF: stages top (S = 0) to bottom; DIT: bottom (S = RL-1) to top.
- * LAZYLAST: the last stage's outputs in [0, 4p) (its butterflies all have
- * twiddle 1). */
-template <int PI, int RL, int S, bool LAZYLAST = false>
+/* Forward group of a compile-time round, inputs below 2p. */
+template <int PI, int RL, int BOUT>
 __device__ __forceinline__ void const_dif(uint64_t *x)
 {
-    if constexpr (S < RL) {
-        const_stage<PI, RL, S, 0, false, LAZYLAST && S == RL - 1>(x);
-        const_dif<PI, RL, S + 1, LAZYLAST>(x);
-    }
+    lz_group<PI, RL, false, true, 2, BOUT>(x, ConstTwF<PI, RL, false>{});
 }
-/* REDUCEDIN: the inputs are below 2p (the first stage, S = RL-1, then skips
- * its reductions). */
-template <int PI, int RL, int S, bool REDUCEDIN = false>
+/* Inverse group of a compile-time round. */
+template <int PI, int RL, int BIN, int BOUT>
 __device__ __forceinline__ void const_dit(uint64_t *x)
 {
-    if constexpr (S >= 0) {
-        const_stage<PI, RL, S, 0, true, REDUCEDIN && S == RL - 1>(x);
-        const_dit<PI, RL, S - 1, false>(x);
-    }
+    lz_group<PI, RL, true, true, BIN, BOUT>(x, ConstTwF<PI, RL, true>{});
 }
 
 /* ---- cross twiddles in the last forward / first inverse round -------------
@@ -343,14 +291,36 @@ __host__ __device__ constexpr uint32_t mid_off(int l)
     return ((1u << l) - (1u << RLL)) + (uint32_t)(l - RLL);
 }
 
+/* Stage twiddles of a round from an LDS table: the round's top stage has
+ * half-size 2^LH and group spacing d = 2^LD; stage S of the group (half-size
+ * 2^(LH-S)) pairs values I and I + half with e = j + (I mod half) * d.
+ * MIDRLL > 0: the fused passes' shared table (mid_off<MIDRLL>), where the
+ * inverse takes -w^(hs-e) = w^-e... as the forward entry at hs - e (NEG). */
+template <int LH, int LD, int R, bool DIT, int MIDRLL>
+struct TabTwF {
+    const TwPair *tw;
+    uint32_t j;
+    static constexpr bool NEG = DIT && MIDRLL != 0;
+    template <int S, int I>
+    __device__ __forceinline__ TwPair get() const
+    {
+        constexpr int half = R >> (S + 1);
+        constexpr int lhs = LH - S;
+        const uint32_t e = j + (uint32_t)((I & (half - 1)) << LD);
+        if constexpr (MIDRLL != 0 && DIT) return tw[mid_off<MIDRLL ? MIDRLL : 1>(lhs) + (1u << lhs) - e];
+        else if constexpr (MIDRLL != 0) return tw[mid_off<MIDRLL ? MIDRLL : 1>(lhs) + e];
+        else return tw[(1u << lhs) - 1 + e];
+    }
+};
+
 /* ---- radix-R rounds over the LDS tile -------------------------------------
  * A DIF round with top stage half-size h = 2^LH and R = 2^RL covers stages
  * h, h/2, .., h/2^(RL-1): the groups {m0 + i*d}, d = h >> (RL-1),
  * m0 = blk*2h + j, j < d. */
 
-/* MIDRLL > 0: the fused passes' shared table (mid_off<MIDRLL>).  LAZYLAST:
- * the smallest round (LD = 0) leaves its last stage's outputs in [0, 4p). */
-template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0, bool LAZYLAST = false>
+/* MIDRLL > 0: the fused passes' shared table (mid_off<MIDRLL>).  Values in
+ * LDS are below 2p before and after the round. */
+template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0>
 __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint32_t logC, const XtwCtx &xc)
 {
     constexpr int R = 1 << RL;
@@ -369,38 +339,33 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
         uint64_t x[R];
 #pragma unroll
         for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
-        if constexpr (LD == 0) {
-            const_dif<PI, RL, 0, LAZYLAST>(x);
-            if constexpr (XT && xt_ahead<LOGS>()) {
+        if constexpr (LD == 0 && XT) {
+            /* the chain's Montgomery products take values below 8p to (0, 2p) */
+            const_dif<PI, RL, 8>(x);
+            if constexpr (xt_ahead<LOGS>()) {
                 chain_apply<PI, RL>(x, xc.ta, xc.tb);
-            } else if constexpr (XT) {
+            } else {
                 /* the outputs of rows m0 + i (m0 = blk << RL) of column col0 + c */
                 const uint32_t r = xc.col0 + c;
                 chain_twiddle<PI, RL>(x, xc.tw, (r * brev_n(m0, LOGS)) << xc.shift,
                                       (r << (LOGS - RL)) << xc.shift);
             }
+        } else if constexpr (LD == 0) {
+            const_dif<PI, RL, 2>(x);
         } else {
-#pragma unroll
-            for (int s = 0; s < RL; s++) {
-                const int half = R >> (s + 1);          /* pair distance in group index */
-                const int lhs = LH - s;                 /* log2 of this stage's half-size */
-#pragma unroll
-                for (int i = 0; i < R; i++) {
-                    if (i & half) continue;
-                    uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);   /* (m mod hs) */
-                    TwPair w = MIDRLL ? twf[mid_off<MIDRLL ? MIDRLL : 1>(lhs) + e] : twf[(1u << lhs) - 1 + e];
-                    dif_bfly<PI>(x[i], x[i + half], w.w, w.wp);
-                }
-            }
+            lz_group<PI, RL, false, false, 2, 2>(x, TabTwF<LH, LD, R, false, MIDRLL>{twf, j});
         }
 #pragma unroll
         for (int i = 0; i < R; i++) lds[LP(base + ((uint32_t)(i << LD) << logC))] = x[i];
     }
 }
 
-/* REDUCEDIN: the smallest round's (LD = 0) inputs are below 2p. */
-/* GSTORE: the round's outputs go to global memory instead of back to LDS,
- * tile word q to gout[q] (a tile contiguous in its row: one column, T = 1). */
+/* Values in LDS are below 4p before and after the round; REDUCEDIN: the
+ * smallest round's (LD = 0) inputs are below 2p.  GSTORE: the round's
+ * outputs go to global memory instead of back to LDS, tile word q to gout[q]
+ * (a tile contiguous in its row: one column, T = 1), unreduced (below 16p:
+ * what reads inverse-pass output reduces it in one step or multiplies it by
+ * a factor below 2p). */
 template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0, bool REDUCEDIN = false, bool GSTORE = false>
 __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint32_t logC, const XtwCtx &xc,
                                           uint64_t *gout = nullptr)
@@ -421,7 +386,9 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
         uint64_t x[R];
 #pragma unroll
         for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
+        constexpr int BOUT = GSTORE ? LZ_MAX : 4;
         if constexpr (LD == 0) {
+            /* the chain's Montgomery products leave any input below 3p */
             if constexpr (XT && xt_ahead<LOGS>()) {
                 chain_apply<PI, RL>(x, xc.ta, xc.tb);
             } else if constexpr (XT) {
@@ -430,25 +397,9 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
                 chain_twiddle<PI, RL>(x, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift),
                                       0u - ((r << (LOGS - RL)) << xc.shift));
             }
-            const_dit<PI, RL, RL - 1, REDUCEDIN && !XT>(x);
+            const_dit<PI, RL, REDUCEDIN && !XT ? 2 : 4, BOUT>(x);
         } else {
-#pragma unroll
-            for (int s = RL - 1; s >= 0; s--) {
-                const int half = R >> (s + 1);
-                const int lhs = LH - s;
-#pragma unroll
-                for (int i = 0; i < R; i++) {
-                    if (i & half) continue;
-                    uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
-                    if constexpr (MIDRLL) {     /* -w^(hs-e): the forward entry, outputs swapped */
-                        const TwPair w = twi[mid_off<MIDRLL ? MIDRLL : 1>(lhs) + (1u << lhs) - e];
-                        dit_bfly_neg<PI>(x[i], x[i + half], w.w, w.wp);
-                    } else {
-                        const TwPair w = twi[(1u << lhs) - 1 + e];
-                        dit_bfly<PI>(x[i], x[i + half], w.w, w.wp);
-                    }
-                }
-            }
+            lz_group<PI, RL, true, false, 4, BOUT>(x, TabTwF<LH, LD, R, true, MIDRLL>{twi, j});
         }
         if constexpr (GSTORE) {
 #pragma unroll
@@ -485,18 +436,8 @@ __device__ __forceinline__ void dif_round_hz(uint64_t *lds, const TwPair *twf, u
             const TwPair w = twf[(1u << LH) - 1 + j + (uint32_t)(i << LD)];
             x[i + R / 2] = shoup_small<PI>((uint32_t)x[i], w.w, w.wp);
         }
-#pragma unroll
-        for (int s = 1; s < RL; s++) {
-            const int half = R >> (s + 1);
-            const int lhs = LH - s;
-#pragma unroll
-            for (int i = 0; i < R; i++) {
-                if (i & half) continue;
-                const uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
-                const TwPair w = twf[(1u << lhs) - 1 + e];
-                dif_bfly<PI>(x[i], x[i + half], w.w, w.wp);
-            }
-        }
+        /* the other two stages: every value below 2p now */
+        lz_group<PI, RL, false, false, 2, 2, 1>(x, TabTwF<LH, LD, R, false, 0>{twf, j});
 #pragma unroll
         for (int i = 0; i < R; i++) lds[LP(base + ((uint32_t)(i << LD) << logC))] = x[i];
     }
@@ -558,7 +499,7 @@ __device__ __forceinline__ void dif_last_store(const PassArgs &pa, const FGeom &
         uint64_t v[R];
 #pragma unroll
         for (int i = 0; i < R; i++) v[i] = lds[LP(q0 + ((uint32_t)i << g.logC))];
-        const_dif<PI, RL, 0>(v);
+        const_dif<PI, RL, 8>(v);     /* the chain takes values below 8p to (0, 2p) */
         if constexpr (RL == 3 && xt_ahead<LOGS>()) {
             chain_apply<PI, 3>(v, xc.ta, xc.tb);
         } else {
@@ -583,32 +524,22 @@ __device__ __forceinline__ void dit_top_store(const uint64_t *lds, const TwPair 
     uint64_t v[R];
 #pragma unroll
     for (int i = 0; i < R; i++) v[i] = lds[LP(tid + i * FTPB)];
-#pragma unroll
-    for (int s = RL - 1; s >= 0; s--) {
-        const int half = R >> (s + 1);
-        const int lhs = LH - s;
-#pragma unroll
-        for (int i = 0; i < R; i++) {
-            if (i & half) continue;
-            const uint32_t e = j + (uint32_t)((i & (half - 1)) << LD);
-            const TwPair w = twi[(1u << lhs) - 1 + e];
-            dit_bfly<PI>(v[i], v[i + half], w.w, w.wp);
-        }
-    }
+    /* stored unreduced, below 16p (inverse-pass output) */
+    lz_group<PI, RL, true, false, 4, LZ_MAX>(v, TabTwF<LH, LD, R, true, 0>{twi, j});
 #pragma unroll
     for (int i = 0; i < R; i++) x[gb + i * gs] = v[i];
 }
 
 /* All DIF rounds of an S = 2^LOGS transform, top stage first, radix 8 while
  * three or more stages remain. */
-template <int PI, int LOGS, int LH, bool XT = false, bool LAZYLAST = false>
+template <int PI, int LOGS, int LH, bool XT = false>
 __device__ __forceinline__ void dif_all(uint64_t *lds, const TwPair *twf, uint32_t logC, const XtwCtx &xc = {})
 {
     if constexpr (LH >= 0) {
         constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
-        dif_round<PI, LOGS, LH, RL, XT, 0, LAZYLAST>(lds, twf, logC, xc);
+        dif_round<PI, LOGS, LH, RL, XT, 0>(lds, twf, logC, xc);
         lds_sync();
-        dif_all<PI, LOGS, LH - RL, XT, LAZYLAST>(lds, twf, logC, xc);
+        dif_all<PI, LOGS, LH - RL, XT>(lds, twf, logC, xc);
     }
 }
 
@@ -843,9 +774,9 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
                     uint32_t c, m0;
                     (void)last_group<LOGS>(pa, g, tcur, tid, n, c, m0, gb, gs);
                 }
-                /* the last stage in [0, 4p): the Montgomery product by a
-                 * table entry below p returns (0, 2p) */
-                const_dif<PI, RL, 0, true>(v + n * R);
+                /* unreduced (below 16p): the Montgomery product by a table
+                 * entry below p returns (0, 2p) */
+                const_dif<PI, RL, LZ_MAX>(v + n * R);
 #pragma unroll
                 for (int i = 0; i < R; i++) x[gb + i * gs] = mont_mul_lazy<PI>(v[n * R + i], w[n * R + i]);
             }
@@ -1104,10 +1035,10 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
      * pointwise product, after b's rounds) */
 #pragma unroll
     for (int i = 0; i < FT_PER; i++) asm volatile("" : "+v"(ra[i]));
-    /* a's values may stay in [0, 4p): the Montgomery product takes one
-     * factor below 4p when the other is below 2p */
+    /* a's values stay unreduced (below 16p): the Montgomery product takes
+     * any first factor when the second is below 2p, and returns (0, 3p) */
 #pragma unroll
-    for (int n = 0; n < NG; n++) const_dif<PI, RLL, 0, true>(ra + n * RR);
+    for (int n = 0; n < NG; n++) const_dif<PI, RLL, LZ_MAX>(ra + n * RR);
 #pragma unroll
     for (int i = 0; i < FT_PER; i++) asm volatile("" : "+v"(ra[i]));
     if (!pa.square) {
@@ -1123,11 +1054,11 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
         uint64_t x[RR];
         if (pa.square) {      /* the second factor below 2p */
 #pragma unroll
-            for (int i = 0; i < RR; i++) x[i] = reduce2<PI>(ra[n * RR + i]);
+            for (int i = 0; i < RR; i++) x[i] = reduce_any<PI>(ra[n * RR + i]);
         } else {
 #pragma unroll
             for (int i = 0; i < RR; i++) x[i] = lds[LP(base + i)];
-            const_dif<PI, RLL, 0>(x);
+            const_dif<PI, RLL, 2>(x);
         }
         if (pa.scale_late) {    /* the last inverse pass applies the 1/L scale */
 #pragma unroll
@@ -1136,7 +1067,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 #pragma unroll
             for (int i = 0; i < RR; i++) x[i] = shoup_lazy<PI>(mont_mul_lazy<PI>(ra[n * RR + i], x[i]), s, sp);
         }
-        const_dit<PI, RLL, RLL - 1>(x);
+        const_dit<PI, RLL, 4, 4>(x);     /* the products are below 3p */
 #pragma unroll
         for (int i = 0; i < RR; i++) lds[LP(base + i)] = x[i];
     }

@@ -4,10 +4,12 @@
  * the documented output range of the Montgomery product, the Shoup product
  * (and its small-operand form), both butterflies (the inverse one also with a
  * negated twiddle), the one-step reduction of any 64-bit value, the
- * compile-time roots, and the CRT tests that three residues are those of one
+ * compile-time roots, the lazy butterfly groups of the transform rounds, and
+ * the CRT tests that three residues are those of one
  * integer below p0, or below 2 p0 (against Garner's digits).
  * Prints "bad N" and exits non-zero on any mismatch.
  */
+#include "../../cryptography_amd/csrc/lazy_group.h"
 #include "../../cryptography_amd/csrc/modarith.h"
 
 #include <stdio.h>
@@ -49,9 +51,9 @@ static void check(long iters)
     auto mont = [&](uint64_t a, uint64_t b) {
         const uint64_t r = mont_mul_lazy<PI>(a, b);
         expect((u128)(r % p) * R % p == (u128)a * b % p, "mont value", PI, a, b);
-        expect(r > 0 && r < 3 * p, "mont range (a<4p, b<2p)", PI, a, b);
-        if (a < 2 * p && b < 2 * p) expect(r < 2 * p, "mont range (a,b<2p)", PI, a, b);
-        if (b < p) expect(r < 2 * p, "mont range (a<4p, b<p)", PI, a, b);
+        expect(r > 0 && r < 3 * p, "mont range (any a, b<2p)", PI, a, b);
+        if (a < 8 * p && b < 2 * p) expect(r < 2 * p, "mont range (a<8p, b<2p)", PI, a, b);
+        if (b < p) expect(r < 2 * p, "mont range (any a, b<p)", PI, a, b);
     };
     auto shoup = [&](uint64_t a, uint64_t w) {
         const uint64_t wp = (uint64_t)(((u128)w << 64) / p);
@@ -100,12 +102,15 @@ static void check(long iters)
     for (long i = 0; i < iters; i++) red(rnd());
     for (uint64_t a : edges)
         for (uint64_t b : edges) {
-            if (a < 4 * p && b < 2 * p) mont(a, b);
+            if (b < 2 * p) mont(a, b);
             if (b < p) shoup(a, b);
             if (b < p && a <= 0xffffffffull) shoup_s((uint32_t)a, b);
             if (a < 2 * p && b < 2 * p) bfly(a, b, (b % p) ? b % p : 1);
         }
     for (long i = 0; i < iters; i++) {
+        mont(rnd(), rnd() % (2 * p));
+        mont(rnd() % (8 * p), rnd() % (2 * p));
+        mont(~0ull - (rnd() & 0xffff), 2 * p - 1 - (rnd() & 0xffff));
         mont(rnd() % (4 * p), rnd() % (2 * p));
         mont(rnd() % (2 * p), rnd() % (2 * p));
         mont(rnd() % (4 * p), rnd() % p);
@@ -119,6 +124,110 @@ static void check(long iters)
     expect(powmod(T::w, 1u << 12, p) == 1 && powmod(T::w, 1u << 11, p) == p - 1, "root order", PI, T::w, 0);
     using T5 = ConstTw<PI, 12, 5>;
     expect(T5::w == powmod(T::w, 5, p) && T5::wp == (uint64_t)(((u128)T5::w << 64) / p), "root power", PI, T5::w, 0);
+}
+
+/* Lazy butterfly groups (lazy_group.h) against the same butterfly network in
+ * exact arithmetic mod p: every plan the kernels use and more, inputs at and
+ * just below their bounds as well as random, random table twiddles (also the
+ * negated-twiddle inverse form) and the compile-time roots.  Each output must
+ * be congruent to the exact one and below its planned bound. */
+struct RandTw {
+    static constexpr bool NEG = false;
+    TwPair t[3][8];
+    template <int S, int I>
+    TwPair get() const { return t[S][I]; }
+};
+struct RandTwNeg : RandTw {
+    static constexpr bool NEG = true;
+};
+
+template <class TW, int RL, int S = 0, int I = 0>
+static void tw_table(const TW &tw, uint64_t (&w)[3][8])
+{
+    if constexpr (S < RL) {
+        if constexpr (I < (1 << RL)) {
+            w[S][I] = tw.template get<S, I>().w;
+            tw_table<TW, RL, S, I + 1>(tw, w);
+        } else {
+            tw_table<TW, RL, S + 1, 0>(tw, w);
+        }
+    }
+}
+
+template <int PI, int RL, bool DIT, bool TW1, int BIN, int BOUT, int J0, class TW>
+static void check_lazy_one(const TW &tw, long iters)
+{
+    const uint64_t p = prime(PI);
+    constexpr int R = 1 << RL;
+    constexpr LzPlan PL = LzRound<RL, DIT, TW1, BIN, BOUT, J0>::plan;
+    uint64_t w[3][8];
+    tw_table<TW, RL>(tw, w);
+    for (long it = 0; it < iters; it++) {
+        uint64_t x[8], e[8];
+        for (int i = 0; i < R; i++) {
+            const uint64_t top = (uint64_t)BIN * p;
+            const int mode = (int)(rnd() % 4);
+            x[i] = mode == 0 ? top - 1 - (rnd() & 0xfff) : mode == 1 ? top - 1 : rnd() % top;
+            e[i] = x[i] % p;
+        }
+        lz_group<PI, RL, DIT, TW1, BIN, BOUT, J0>(x, tw);
+        for (int j = J0; j < RL; j++) {
+            const int S = DIT ? RL - 1 - j : j, half = R >> (S + 1);
+            for (int i = 0; i < R; i++) {
+                if (i & half) continue;
+                const bool one = TW1 && (i & (half - 1)) == 0;
+                const uint64_t wv = one ? 1 : w[S][i], a = e[i], b = e[i + half];
+                if (!DIT) {
+                    e[i] = (a + b) % p;
+                    e[i + half] = (uint64_t)((u128)((a + p - b) % p) * wv % p);
+                } else {
+                    const uint64_t t = (uint64_t)((u128)b * wv % p);
+                    const bool neg = TW::NEG && !one;
+                    e[i] = neg ? (a + p - t) % p : (a + t) % p;
+                    e[i + half] = neg ? (a + t) % p : (a + p - t) % p;
+                }
+            }
+        }
+        for (int i = 0; i < R; i++) {
+            expect(x[i] % p == e[i], "lazy group value", PI, (uint64_t)(RL * 1000 + DIT * 100 + TW1 * 10 + i), x[i]);
+            expect(PL.out[i] <= BOUT && (u128)x[i] < (u128)PL.out[i] * p, "lazy group bound", PI,
+                   (uint64_t)(RL * 1000 + DIT * 100 + TW1 * 10 + i), x[i]);
+        }
+    }
+}
+
+template <int PI, int RL>
+static void check_lazy_rl(long iters)
+{
+    const uint64_t p = prime(PI);
+    RandTw tw;
+    RandTwNeg twn;
+    for (int s = 0; s < 3; s++)
+        for (int i = 0; i < 8; i++) {
+            const uint64_t v = (s + i) % 3 == 0 ? p - 1 : rnd() % p;
+            tw.t[s][i] = twn.t[s][i] = TwPair{v, (uint64_t)(((u128)v << 64) / p)};
+        }
+    check_lazy_one<PI, RL, false, false, 2, 2, 0>(tw, iters);
+    check_lazy_one<PI, RL, false, true, 2, 2, 0>(ConstTwF<PI, RL, false>{}, iters);
+    check_lazy_one<PI, RL, false, true, 2, 8, 0>(ConstTwF<PI, RL, false>{}, iters);
+    check_lazy_one<PI, RL, false, true, 2, LZ_MAX, 0>(ConstTwF<PI, RL, false>{}, iters);
+    check_lazy_one<PI, RL, true, false, 4, 4, 0>(tw, iters);
+    check_lazy_one<PI, RL, true, false, 4, LZ_MAX, 0>(tw, iters);
+    check_lazy_one<PI, RL, true, false, 4, 4, 0>(twn, iters);
+    check_lazy_one<PI, RL, true, false, 4, LZ_MAX, 0>(twn, iters);
+    check_lazy_one<PI, RL, true, true, 2, 4, 0>(ConstTwF<PI, RL, true>{}, iters);
+    check_lazy_one<PI, RL, true, true, 4, 4, 0>(ConstTwF<PI, RL, true>{}, iters);
+    check_lazy_one<PI, RL, true, true, 4, LZ_MAX, 0>(ConstTwF<PI, RL, true>{}, iters);
+    check_lazy_one<PI, RL, true, false, LZ_MAX, 4, 0>(tw, iters);
+    if constexpr (RL == 3) check_lazy_one<PI, RL, false, false, 2, 2, 1>(tw, iters);
+}
+
+template <int PI>
+static void check_lazy(long iters)
+{
+    check_lazy_rl<PI, 1>(iters);
+    check_lazy_rl<PI, 2>(iters);
+    check_lazy_rl<PI, 3>(iters);
 }
 
 /* crt_below_p0 against the textbook Garner digits (128-bit arithmetic):
@@ -185,6 +294,8 @@ int main(int argc, char **argv)
     check<0>(iters);
     check<1>(iters);
     check<2>(iters);
+    check_lazy<0>(iters / 20);
+    check_lazy<2>(iters / 20);
     check_crt(iters / 8);
     printf("bad %d\n", bad);
     return bad != 0;
