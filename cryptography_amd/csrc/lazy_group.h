@@ -146,6 +146,26 @@ __host__ __device__ __forceinline__ void lz_step(uint64_t *x, const TW &tw)
     }
 }
 
+/* The largest bound a plan of 2^RL values leaves. */
+__host__ __device__ constexpr int lz_out_max(const LzPlan &P, int RL)
+{
+    int m = 0;
+    for (int i = 0; i < (1 << RL); i++) m = P.out[i] > m ? P.out[i] : m;
+    return m;
+}
+
+/* Inverse rounds pass their values on unreduced (below 16p).  The bound of
+ * the values after the inverse rounds of a sub-transform up to top
+ * half-size 2^LH (radix 8 from the top, the smallest round with compile-time
+ * twiddles), the smallest round's inputs below BIN0*p; LH < 0: BIN0. */
+__host__ __device__ constexpr int dit_bound(int LH, int bin0)
+{
+    if (LH < 0) return bin0;
+    const int RL = LH + 1 >= 3 ? 3 : LH + 1;
+    const int prev = dit_bound(LH - RL, bin0);
+    return lz_out_max(lz_plan(RL, true, LH - (RL - 1) == 0, prev, LZ_MAX, 0), RL);
+}
+
 /* All steps of one group: inputs below BIN*p, outputs below BOUT*p. */
 template <int PI, int RL, bool DIT, bool TW1, int BIN, int BOUT, int J0 = 0, class TW>
 __host__ __device__ __forceinline__ void lz_group(uint64_t *x, const TW &tw)

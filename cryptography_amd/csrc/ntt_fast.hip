@@ -360,13 +360,13 @@ __device__ __forceinline__ void dif_round(uint64_t *lds, const TwPair *twf, uint
     }
 }
 
-/* Values in LDS are below 4p before and after the round; REDUCEDIN: the
- * smallest round's (LD = 0) inputs are below 2p.  GSTORE: the round's
- * outputs go to global memory instead of back to LDS, tile word q to gout[q]
- * (a tile contiguous in its row: one column, T = 1), unreduced (below 16p:
- * what reads inverse-pass output reduces it in one step or multiplies it by
- * a factor below 2p). */
-template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL = 0, bool REDUCEDIN = false, bool GSTORE = false>
+/* Values leave unreduced, below 16p (what reads inverse-pass output reduces
+ * it in one step or multiplies it by a factor below 2p); BIN0: the
+ * smallest round's (LD = 0) inputs are below BIN0*p, and every other
+ * round's inputs below the bound the rounds under it leave (dit_bound).
+ * GSTORE: the round's outputs go to global memory instead of back to LDS,
+ * tile word q to gout[q] (a tile contiguous in its row: one column, T = 1). */
+template <int PI, int LOGS, int LH, int RL, bool XT, int MIDRLL, int BIN0, bool GSTORE = false>
 __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint32_t logC, const XtwCtx &xc,
                                           uint64_t *gout = nullptr)
 {
@@ -386,7 +386,6 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
         uint64_t x[R];
 #pragma unroll
         for (int i = 0; i < R; i++) x[i] = lds[LP(base + ((uint32_t)(i << LD) << logC))];
-        constexpr int BOUT = GSTORE ? LZ_MAX : 4;
         if constexpr (LD == 0) {
             /* the chain's Montgomery products leave any input below 3p */
             if constexpr (XT && xt_ahead<LOGS>()) {
@@ -397,9 +396,9 @@ __device__ __forceinline__ void dit_round(uint64_t *lds, const TwPair *twi, uint
                 chain_twiddle<PI, RL>(x, xc.tw, 0u - ((r * brev_n(m0, LOGS)) << xc.shift),
                                       0u - ((r << (LOGS - RL)) << xc.shift));
             }
-            const_dit<PI, RL, REDUCEDIN && !XT ? 2 : 4, BOUT>(x);
+            const_dit<PI, RL, XT ? 4 : BIN0, LZ_MAX>(x);
         } else {
-            lz_group<PI, RL, true, false, 4, BOUT>(x, TabTwF<LH, LD, R, true, MIDRLL>{twi, j});
+            lz_group<PI, RL, true, false, dit_bound(LH - RL, BIN0), LZ_MAX>(x, TabTwF<LH, LD, R, true, MIDRLL>{twi, j});
         }
         if constexpr (GSTORE) {
 #pragma unroll
@@ -514,7 +513,7 @@ __device__ __forceinline__ void dif_last_store(const PassArgs &pa, const FGeom &
 /* The top DIT round of a C * S = 4096 tile (radix 8 for every S >= 8),
  * stored from registers: thread tid's group is tile words tid + k*FTPB,
  * which tile_span places at gb + k*gs. */
-template <int PI, int LOGS>
+template <int PI, int LOGS, int BIN>
 __device__ __forceinline__ void dit_top_store(const uint64_t *lds, const TwPair *twi, uint32_t logC, uint64_t *x,
                                               size_t gb, size_t gs)
 {
@@ -524,8 +523,8 @@ __device__ __forceinline__ void dit_top_store(const uint64_t *lds, const TwPair 
     uint64_t v[R];
 #pragma unroll
     for (int i = 0; i < R; i++) v[i] = lds[LP(tid + i * FTPB)];
-    /* stored unreduced, below 16p (inverse-pass output) */
-    lz_group<PI, RL, true, false, 4, LZ_MAX>(v, TabTwF<LH, LD, R, true, 0>{twi, j});
+    /* inputs below BIN*p, stored unreduced, below 16p (inverse-pass output) */
+    lz_group<PI, RL, true, false, BIN, LZ_MAX>(v, TabTwF<LH, LD, R, true, 0>{twi, j});
 #pragma unroll
     for (int i = 0; i < R; i++) x[gb + i * gs] = v[i];
 }
@@ -550,7 +549,7 @@ __device__ __forceinline__ void dit_all(uint64_t *lds, const TwPair *twi, uint32
     if constexpr (LH >= 0) {
         constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
         dit_all<PI, LOGS, LH - RL, XT, REDUCEDIN>(lds, twi, logC, xc);
-        dit_round<PI, LOGS, LH, RL, XT, 0, REDUCEDIN>(lds, twi, logC, xc);
+        dit_round<PI, LOGS, LH, RL, XT, 0, REDUCEDIN && !XT ? 2 : 4>(lds, twi, logC, xc);
         lds_sync();
     }
 }
@@ -896,7 +895,7 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
             asm volatile("" : "+v"(tid));
             size_t base, step;
             tile_span<LOGS>(pa, g, tt, tid, base, step);
-            dit_top_store<PI, LOGS>(lds, twi, g.logC, x, base, step);
+            dit_top_store<PI, LOGS, dit_bound(LOGS - 4, XTAB ? 2 : 4)>(lds, twi, g.logC, x, base, step);
         } else {
             dit_all<PI, LOGS, LOGS - 1, !LOADCHAIN, XTAB>(lds, twi, g.logC, xc);
             tid = threadIdx.x;
@@ -939,7 +938,7 @@ __device__ __forceinline__ void dit_upper(uint64_t *lds, const TwPair *tw, uint6
     constexpr int RL = LH + 1 >= 3 ? 3 : LH + 1;
     if constexpr (LH - RL >= 0) {
         dit_upper<PI, LOGS, LH - RL, RLL, WL, false>(lds, tw);
-        dit_round<PI, LOGS, LH, RL, false, RLL, false, GSTORE>(lds, tw, 0, XtwCtx{}, gout);
+        dit_round<PI, LOGS, LH, RL, false, RLL, 4, GSTORE>(lds, tw, 0, XtwCtx{}, gout);
         if constexpr (!GSTORE) round_sync<WL>();
     }
 }
@@ -1067,7 +1066,7 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 #pragma unroll
             for (int i = 0; i < RR; i++) x[i] = shoup_lazy<PI>(mont_mul_lazy<PI>(ra[n * RR + i], x[i]), s, sp);
         }
-        const_dit<PI, RLL, 4, 4>(x);     /* the products are below 3p */
+        const_dit<PI, RLL, 4, LZ_MAX>(x);     /* the products are below 3p */
 #pragma unroll
         for (int i = 0; i < RR; i++) lds[LP(base + i)] = x[i];
     }

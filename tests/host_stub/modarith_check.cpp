@@ -219,6 +219,14 @@ static void check_lazy_rl(long iters)
     check_lazy_one<PI, RL, true, true, 4, 4, 0>(ConstTwF<PI, RL, true>{}, iters);
     check_lazy_one<PI, RL, true, true, 4, LZ_MAX, 0>(ConstTwF<PI, RL, true>{}, iters);
     check_lazy_one<PI, RL, true, false, LZ_MAX, 4, 0>(tw, iters);
+    /* inverse rounds chained unreduced: inputs up to 16p */
+    check_lazy_one<PI, RL, true, false, 6, LZ_MAX, 0>(tw, iters);
+    check_lazy_one<PI, RL, true, false, 8, LZ_MAX, 0>(twn, iters);
+    check_lazy_one<PI, RL, true, false, 10, LZ_MAX, 0>(tw, iters);
+    check_lazy_one<PI, RL, true, false, 12, LZ_MAX, 0>(twn, iters);
+    check_lazy_one<PI, RL, true, false, 14, LZ_MAX, 0>(tw, iters);
+    check_lazy_one<PI, RL, true, false, LZ_MAX, LZ_MAX, 0>(twn, iters);
+    check_lazy_one<PI, RL, true, true, 4, LZ_MAX, 0>(ConstTwF<PI, RL, true>{}, iters);
     if constexpr (RL == 3) check_lazy_one<PI, RL, false, false, 2, 2, 1>(tw, iters);
 }
 
