@@ -84,7 +84,6 @@ __host__ __device__ __forceinline__ uint64_t mulhi64(uint64_t a, uint64_t b)
 template <int PI>
 struct Prime {
     static constexpr uint64_t P = prime(PI);
-    static constexpr uint64_t P2 = 2 * prime(PI);
     static constexpr uint64_t NINV = neg_inv64(prime(PI));   /* -p^-1 mod 2^64 */
 };
 
@@ -94,13 +93,6 @@ __host__ __device__ __forceinline__ uint64_t reduce1(uint64_t x)
 {
     return x >= Prime<PI>::P ? x - Prime<PI>::P : x;
 }
-/* x in [0, 4p) -> [0, 2p) */
-template <int PI>
-__host__ __device__ __forceinline__ uint64_t reduce2(uint64_t x)
-{
-    return x >= Prime<PI>::P2 ? x - Prime<PI>::P2 : x;
-}
-
 /* any x < 2^64 -> [0, 2p), congruent: x - q*p for q = floor(x / 2^60) < 16.
  * With x = q*2^60 + s (s < 2^60) and d = 2^60 - p = c*2^32 - 1, the result is
  * s + q*d < 2^60 + 15d < 2p (as 17d < 2^60), and it is >= 0 as q*p <= x.
@@ -220,16 +212,6 @@ struct ConstTw {
     static constexpr uint64_t wp = (uint64_t)(((unsigned __int128)w << 64) / P);
 };
 
-/* Forward butterfly with twiddle 1: (x, y) -> (x + y, x - y), [0,2p) in and out. */
-template <int PI>
-__host__ __device__ __forceinline__ void bfly_one(uint64_t &x, uint64_t &y)
-{
-    uint64_t s = x + y;
-    uint64_t d = x - y + Prime<PI>::P2;
-    x = reduce2<PI>(s);
-    y = reduce2<PI>(d);
-}
-
 /* Montgomery product a*b*2^-64 mod p for any a < 2^64 and b < 2p; result in
  * (T/2^64, T/2^64 + p] for T = a*b (derivation below): below 3p always,
  * below 2p when T < 2^64 p (b < p, or a < 2^63), below p + 1 when T < 2^64.
@@ -266,52 +248,6 @@ __host__ __device__ __forceinline__ uint64_t mont_mul_lazy(uint64_t a, uint64_t 
     const uint32_t dhi = (uint32_t)(dd >> 32) - l0;                /* floor(D / 2^32), two's complement */
     const uint64_t x = hi + (uint64_t)(int64_t)(int32_t)dhi + P;
     return (uint64_t)s0 * NPH + x - ((uint64_t)s0 << 32);          /* x - s0*ph */
-}
-
-/* Forward (decimation-in-frequency) butterfly on [0,2p) values:
- * (x, y) -> (x + y, (x - y) * w), both outputs in [0, 2p). */
-template <int PI>
-__host__ __device__ __forceinline__ void dif_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wp)
-{
-    uint64_t s = x + y;
-    uint64_t d = x - y + Prime<PI>::P2;
-    x = reduce2<PI>(s);
-    y = shoup_lazy<PI>(d, w, wp);
-}
-
-/* Inverse (decimation-in-time) butterfly in Harvey's [0,4p) form:
- * (x, y) -> (x + y*w, x - y*w) for x, y in [0, 4p), both outputs in [0, 4p).
- * Only x is reduced, to [0, 2p); y*w comes
- * from Shoup in [0, 2p) for any y.  Inverse passes therefore leave residues
- * in [0, 4p); whatever reads them last reduces twice. */
-template <int PI>
-__host__ __device__ __forceinline__ void dit_bfly(uint64_t &x, uint64_t &y, uint64_t w, uint64_t wp)
-{
-    const uint64_t t = shoup_lazy<PI>(y, w, wp);
-    const uint64_t u = reduce2<PI>(x);
-    x = u + t;
-    y = u - t + Prime<PI>::P2;
-}
-
-/* The same butterfly for twiddle w^-e given W = w^(hs-e) = -w^-e (w of order
- * 2hs): y*w^-e = -y*W, so the sum and difference trade places.  Ranges as
- * dit_bfly. */
-template <int PI>
-__host__ __device__ __forceinline__ void dit_bfly_neg(uint64_t &x, uint64_t &y, uint64_t W, uint64_t Wp)
-{
-    const uint64_t t = shoup_lazy<PI>(y, W, Wp);
-    const uint64_t u = reduce2<PI>(x);
-    x = u - t + Prime<PI>::P2;
-    y = u + t;
-}
-
-/* The same with twiddle 1: both inputs reduced to [0, 2p). */
-template <int PI>
-__host__ __device__ __forceinline__ void dit_bfly_one(uint64_t &x, uint64_t &y)
-{
-    const uint64_t u = reduce2<PI>(x), t = reduce2<PI>(y);
-    x = u + t;
-    y = u - t + Prime<PI>::P2;
 }
 
 }  // namespace nttc

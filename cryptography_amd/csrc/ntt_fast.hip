@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(256) k_xtab_gen(const TwPair *tw, uint64_t *ou
 }
 
 /* x[i] *= w^(EA + bitrev_RL(i) * EB) for i < 2^RL (exponents mod 2^30);
- * x in [0, 2p) gives [0, 2p), x in [0, 4p) (inverse passes) gives [0, 4p). */
+ * the Montgomery products leave x below 8p in (0, 2p) and any x in (0, 3p). */
 template <int PI, int RL>
 __device__ __forceinline__ void chain_apply(uint64_t *x, uint64_t t, const uint64_t b)
 {
@@ -282,7 +282,7 @@ __device__ __forceinline__ void xt_lookup(XtwCtx &xc, uint32_t logC)
  * the smallest RLL stages use compile-time twiddles) keeps w_(2hs)^k for
  * k = 0 .. hs at [mid_off(l), mid_off(l) + hs]: the forward twiddle w^e at
  * +e, and the inverse twiddle w^-e = -w^(hs-e) as the entry at +(hs - e)
- * with the butterfly's outputs swapped (dit_bfly_neg; +hs holds w^hs = -1,
+ * with the butterfly's outputs swapped (TabTwF::NEG; +hs holds w^hs = -1,
  * for e = 0).  510 entries for S = 512: with the 32 KB tile a block needs
  * 40 KB of LDS, and four blocks share a CU. */
 template <int RLL>
@@ -294,8 +294,9 @@ __host__ __device__ constexpr uint32_t mid_off(int l)
 /* Stage twiddles of a round from an LDS table: the round's top stage has
  * half-size 2^LH and group spacing d = 2^LD; stage S of the group (half-size
  * 2^(LH-S)) pairs values I and I + half with e = j + (I mod half) * d.
- * MIDRLL > 0: the fused passes' shared table (mid_off<MIDRLL>), where the
- * inverse takes -w^(hs-e) = w^-e... as the forward entry at hs - e (NEG). */
+ * MIDRLL > 0: the fused passes' shared table (mid_off<MIDRLL>); there the
+ * inverse twiddle w^-e is read as the forward entry at hs - e, which holds
+ * w^(hs-e) = -w^-e, so the inverse butterfly's outputs trade places (NEG). */
 template <int LH, int LD, int R, bool DIT, int MIDRLL>
 struct TabTwF {
     const TwPair *tw;

@@ -44,14 +44,14 @@ __device__ uint64_t powmod_dev(uint64_t b, uint64_t e, uint64_t r2)
     return r;
 }
 
-/* floor(w * 2^64 / p) for w < p < 2^62 by restoring long division. */
+/* floor(w * 2^64 / p) for w < p < 2^60 by restoring long division. */
 template <int PI>
 __device__ uint64_t shoup_partner(uint64_t w)
 {
     const uint64_t p = Prime<PI>::P;
     uint64_t r = w, q = 0;
     for (int i = 63; i >= 0; i--) {
-        r <<= 1;                       /* r < p < 2^62: no overflow */
+        r <<= 1;                       /* r < p < 2^60: no overflow */
         if (r >= p) { r -= p; q |= (uint64_t)1 << i; }
     }
     return q;

@@ -6,8 +6,8 @@
  * baseline leg do, and only as the checker (or as the timed CPU baseline).
  *
  * Every function cites the reference file:line whose behaviour it restates.
- * The arithmetic parameters follow the build spec (three primes c*2^k+1 below
- * 2^62, 16-bit coefficients) rather than the reference's four 2^64-2^b+1
+ * The arithmetic parameters follow the build spec (three primes c*2^k+1 just
+ * below 2^60, 16-bit coefficients) rather than the reference's four 2^64-2^b+1
  * primes and whole-limb coefficients; the algorithms are the same.
  *
  * Pinning (what checks each function independently of this file):

@@ -1,9 +1,10 @@
-"""Modular primitives of the transform kernels (cryptography_amd/csrc/modarith.h),
-compiled for the host and checked against 128-bit integer arithmetic: values and
-the lazy output ranges the kernels rely on (Montgomery product (0, 3p) / (0, 2p),
-Shoup [0, 2p), DIF butterfly [0, 2p), DIT butterfly [0, 4p)), and the
-compile-time roots.  The reference's counterparts are ntt/lib/primes.h:193-279
-and ntt/lib/shoup.h:45-112."""
+"""Modular primitives of the transform kernels (cryptography_amd/csrc/modarith.h,
+lazy_group.h), compiled for the host and checked against 128-bit integer
+arithmetic: values and the output ranges the kernels rely on (Montgomery
+product (0, 3p) / (0, 2p), Shoup [0, 2p), the one-step reduction of any 64-bit
+value), every lazy butterfly group plan the transform rounds use, with inputs
+at their bounds, the compile-time roots and the CRT checks.  The reference's
+counterparts are ntt/lib/primes.h:193-279 and ntt/lib/shoup.h:45-112."""
 import os
 import shutil
 import subprocess

@@ -90,7 +90,7 @@ def test_modops_random_and_corners():
             assert L.orc_mulmod(a, b, p) == a * b % p
             wp = L.orc_shoup_pre(b, p)
             assert wp == (b << 64) // p
-            # Shoup accepts any a < 2^64 (lazy inputs up to 4p)
+            # Shoup accepts any a < 2^64 (lazy inputs up to 16p)
             for x in (a, a + p, a + 3 * p, 2 ** 64 - 1):
                 if x >= 2 ** 64:
                     continue
