@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -708,9 +709,16 @@ int plan_batch(const ntt_mul_desc *h, size_t n, BatchPlan &bp)
          * P = P_L B_R + P_R) */
         const MulDesc *cd = bp.sorted.data() + c.first;
         auto period_ok = [&](uint32_t per) {
-            for (uint32_t i = per; i < c.count; i++)
-                if (cd[i].b != cd[i % per].b || cd[i].nb != cd[i % per].nb) return false;
-            return true;
+            /* cd[i] against cd[i - per]: the same as against cd[i mod per] */
+            std::atomic<bool> ok{true};
+            par_chunks(c.count - per, [&](size_t lo, size_t hi, unsigned) {
+                for (size_t i = per + lo; i < per + hi && ok.load(std::memory_order_relaxed); i++)
+                    if (cd[i].b != cd[i - per].b || cd[i].nb != cd[i - per].nb) {
+                        ok = false;
+                        return;
+                    }
+            });
+            return ok.load();
         };
         c.b_period = c.count;
         if (c.count > 1 && period_ok(1)) c.b_period = 1;

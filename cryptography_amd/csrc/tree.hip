@@ -83,10 +83,21 @@ struct TreeNode {
 };
 
 /* Upper bound on the limbs of B = prod_{k=a}^{b-1} k (and of P < 2B). */
+/* ln((x-1)!) = lgamma(x): Stirling's series for x >= 16 (the dropped terms
+ * are below 1/(1680 x^7) < 1e-11), the library function below. */
+static double ln_gamma(double x)
+{
+    if (x < 16.0) return lgamma(x);
+    const double r = 1.0 / x, r2 = r * r;
+    return (x - 0.5) * log(x) - x + 0.91893853320467274 + r * (1.0 / 12 - r2 * (1.0 / 360 - r2 * (1.0 / 1260)));
+}
+
+/* Limbs that hold B = a (a+1) .. (b-1): log2 B = (ln (b-1)! - ln (a-1)!) / ln 2,
+ * with a relative margin for rounding and two spare limbs. */
 static uint32_t cap_limbs_B(long a, long b)
 {
     if (b <= a) return 1;
-    double bits = (lgamma((double)b) - lgamma((double)a)) / log(2.0);
+    double bits = (ln_gamma((double)b) - ln_gamma((double)a)) / log(2.0);
     return (uint32_t)(bits / 64.0 + 1e-9 * bits) + 2;
 }
 
@@ -183,20 +194,52 @@ int ntt_split_tree_create(long a, long b, int use_graph, ntt_split_tree **out)
      * (A) heights, capacities, and per chunk its arena words, leaves and
      * products per height; (B) arena offsets and the leaf descriptors;
      * (C) the product descriptors, which read the children's offsets. */
-    std::vector<long> lo{a};
-    std::vector<uint32_t> sz{(uint32_t)std::max<long>(b - a, 0)};
-    std::vector<uint32_t> kid{0};
-    for (size_t d0 = 0, d1 = 1; d0 < d1; d0 = d1, d1 = lo.size()) {
-        for (size_t i = d0; i < d1; i++) {
-            const uint32_t s_ = sz[i];
-            if (s_ <= LEAF_TERMS) continue;
-            kid[i] = (uint32_t)lo.size();
-            lo.push_back(lo[i]);                   /* [a, m), m = a + (b-a)/2 */
-            sz.push_back(s_ / 2);
-            lo.push_back(lo[i] + s_ / 2);
-            sz.push_back(s_ - s_ / 2);
-            kid.push_back(0);
-            kid.push_back(0);
+    std::vector<long, uninit_alloc<long>> lo(1, a);
+    std::vector<uint32_t, uninit_alloc<uint32_t>> sz(1, (uint32_t)std::max<long>(b - a, 0)), kid(1, 0);
+    {
+        /* one depth at a time: count the internal nodes per chunk, then each
+         * chunk writes its nodes' children from its own offset.  Leaves hold
+         * at least LEAF_TERMS / 2 terms (or the whole range), so there are at
+         * most 2 (b - a) / (LEAF_TERMS / 2) + 2 nodes. */
+        const size_t cap = 4 * (size_t)std::max<long>(b - a, 0) / LEAF_TERMS + 2;
+        lo.reserve(cap);
+        sz.reserve(cap);
+        kid.reserve(cap);
+        std::vector<size_t> cnt(host_threads() + 1, 0);
+        for (size_t d0 = 0, d1 = 1; d0 < d1;) {
+            const size_t n = d1 - d0;
+            const unsigned nc = par_chunks(n, [&](size_t i0, size_t i1, unsigned w) {
+                size_t c = 0;
+                for (size_t i = d0 + i0; i < d0 + i1; i++) c += sz[i] > LEAF_TERMS;
+                cnt[w] = c;
+            });
+            size_t m = 0;
+            for (unsigned w = 0; w < nc; w++) {
+                const size_t c = cnt[w];
+                cnt[w] = m;
+                m += c;
+            }
+            lo.resize(d1 + 2 * m);
+            sz.resize(d1 + 2 * m);
+            kid.resize(d1 + 2 * m);
+            par_chunks(n, [&](size_t i0, size_t i1, unsigned w) {
+                size_t at = d1 + 2 * cnt[w];
+                for (size_t i = d0 + i0; i < d0 + i1; i++) {
+                    const uint32_t s_ = sz[i];
+                    if (s_ <= LEAF_TERMS) {
+                        kid[i] = 0;
+                        continue;
+                    }
+                    kid[i] = (uint32_t)at;
+                    lo[at] = lo[i];                    /* [a, m), m = a + (b-a)/2 */
+                    sz[at] = s_ / 2;
+                    lo[at + 1] = lo[i] + s_ / 2;
+                    sz[at + 1] = s_ - s_ / 2;
+                    at += 2;
+                }
+            });
+            d0 = d1;
+            d1 += 2 * m;
         }
     }
     const size_t N = lo.size();
