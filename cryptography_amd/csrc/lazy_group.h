@@ -30,12 +30,17 @@ namespace nttc {
  * of 12.  The plan of a group is computed by the compiler (lz_plan) and
  * every reduction and multiple of p in the code is a compile-time choice. */
 #define LZ_MAX 16
+#ifndef NTT_LOOSE_SHOUP
+#define NTT_LOOSE_SHOUP 1
+#endif
 
 struct LzPlan {
     unsigned char k[3][8];     /* step j, x-side index i: the multiple of p in the difference */
     bool pre[3][8];            /* reduce value i before step j */
     bool post[8];              /* reduce value i after the last step */
+    bool loose[3];             /* step j's products use shoup_lazy4 (below 4p) */
     unsigned char out[8];      /* bound of value i at the end */
+    int cost;                  /* reductions * 5 - loose products * 4 (issue cycles, roughly) */
     bool ok;
 };
 
@@ -44,13 +49,15 @@ struct LzPlan {
  * index i with i mod half = 0); table rounds have none.  Steps before J0 are
  * skipped (done otherwise by the caller); values enter below BIN*p and leave
  * below BOUT*p. */
-__host__ __device__ constexpr LzPlan lz_plan(int RL, bool dit, bool tw1, int bin, int bout, int j0)
+__host__ __device__ constexpr LzPlan lz_plan_mask(int RL, bool dit, bool tw1, int bin, int bout, int j0,
+                                                  unsigned loosemask)
 {
     LzPlan P{};
     int b[8] = {};
     const int R = 1 << RL;
     for (int i = 0; i < R; i++) b[i] = bin;
     P.ok = bin <= LZ_MAX && bout >= 2 && RL >= 1 && RL <= 3;
+    for (int j = 0; j < 3; j++) P.loose[j] = !dit && j < RL && ((loosemask >> j) & 1);
     for (int j = j0; j < RL; j++) {
         const int S = dit ? RL - 1 - j : j;
         const int half = R >> (S + 1);
@@ -61,25 +68,25 @@ __host__ __device__ constexpr LzPlan lz_plan(int RL, bool dit, bool tw1, int bin
             if (dit && !one) {
                 if (b[i] + 2 > LZ_MAX) {
                     P.pre[j][i] = true;
+                    P.cost += 5;
                     b[i] = 2;
                 }
                 P.k[j][i] = 2;
                 b[i] = b[y] = b[i] + 2;
             } else {
-                if (b[i] + b[y] > LZ_MAX) {
-                    const int r = b[i] >= b[y] ? i : y;
-                    P.pre[j][r] = true;
-                    b[r] = 2;
-                }
-                if (b[i] + b[y] > LZ_MAX) {
-                    const int r = b[i] >= b[y] ? i : y;
-                    P.pre[j][r] = true;
-                    b[r] = 2;
+                for (int t = 0; t < 2; t++) {
+                    if (b[i] + b[y] > LZ_MAX) {
+                        const int r = b[i] >= b[y] ? i : y;
+                        P.pre[j][r] = true;
+                        P.cost += 5;
+                        b[r] = 2;
+                    }
                 }
                 P.k[j][i] = (unsigned char)b[y];
                 const int sum = b[i] + b[y];
                 b[i] = sum;
-                b[y] = one ? sum : 2;
+                b[y] = one ? sum : P.loose[j] ? 4 : 2;
+                if (!one && P.loose[j]) P.cost -= 4;
             }
             P.ok = P.ok && b[i] <= LZ_MAX && b[y] <= LZ_MAX;
         }
@@ -87,11 +94,34 @@ __host__ __device__ constexpr LzPlan lz_plan(int RL, bool dit, bool tw1, int bin
     for (int i = 0; i < R; i++) {
         if (b[i] > bout) {
             P.post[i] = true;
+            P.cost += 5;
             b[i] = 2;
         }
         P.out[i] = (unsigned char)b[i];
     }
     return P;
+}
+
+/* The plan of least cost over the choices of loose forward steps.  A loose
+ * step's products (below 4p) feed later butterflies of the group, which take
+ * values up to 16p; where the group's outputs must leave below 2p (LDS and
+ * HBM), loose products in the last step would each cost a reduction, so the
+ * cost decides.  Inverse steps stay exact: an inverse product t enters both
+ * outputs, x + t and x - t + k p, so a loose one raises every bound
+ * downstream.  Ties keep fewer loose steps. */
+__host__ __device__ constexpr LzPlan lz_plan(int RL, bool dit, bool tw1, int bin, int bout, int j0)
+{
+    LzPlan best = lz_plan_mask(RL, dit, tw1, bin, bout, j0, 0);
+#if NTT_LOOSE_SHOUP
+    if (!dit) {
+        for (unsigned m = 1; m < (1u << RL); m++) {
+            if (m & ((1u << j0) - 1)) continue;
+            const LzPlan c = lz_plan_mask(RL, dit, tw1, bin, bout, j0, m);
+            if (c.ok && c.cost < best.cost) best = c;
+        }
+    }
+#endif
+    return best;
 }
 
 template <int RL, bool DIT, bool TW1, int BIN, int BOUT, int J0>
@@ -125,7 +155,8 @@ __host__ __device__ __forceinline__ void lz_step(uint64_t *x, const TW &tw)
                 } else if constexpr (!DIT) {
                     const TwPair w = tw.template get<S, I>();
                     x[I] = a + b;
-                    x[Y] = shoup_lazy<PI>(a - b + KP, w.w, w.wp);
+                    if constexpr (RD::plan.loose[J]) x[Y] = shoup_lazy4<PI>(a - b + KP, w.w, w.wp);
+                    else x[Y] = shoup_lazy<PI>(a - b + KP, w.w, w.wp);
                 } else {
                     const TwPair w = tw.template get<S, I>();
                     const uint64_t t = shoup_lazy<PI>(b, w.w, w.wp);

@@ -126,6 +126,26 @@ __host__ __device__ __forceinline__ uint64_t shoup_lazy(uint64_t a, uint64_t w, 
     return a * w - q * Prime<PI>::P;
 }
 
+/* Shoup product a*w mod p with a truncated quotient, for any a < 2^64, w < p,
+ * wp = floor(w*2^64/p); result in [0, 4p).  With a = a1*2^32 + a0 and
+ * wp = wp1*2^32 + wp0, floor(a*wp / 2^64) = a1*wp1 + floor((A + B + C) / 2^32)
+ * for A = a1*wp0, B = a0*wp1, C = hi32(a0*wp0).  q = a1*wp1 + hi32(A) + hi32(B)
+ * drops (A mod 2^32) + (B mod 2^32) + C < 3*2^32, so it is at most two below
+ * floor(a*wp / 2^64), itself floor(a*w/p) or one less: a*w - q*p < 4p.  Eight
+ * 32-bit multiplies and no carry (shoup_lazy: nine and a carry). */
+template <int PI>
+__host__ __device__ __forceinline__ uint64_t shoup_lazy4(uint64_t a, uint64_t w, uint64_t wp)
+{
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), w0 = (uint32_t)wp, w1 = (uint32_t)(wp >> 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint64_t h = (uint64_t)__umulhi(a1, w0) + __umulhi(a0, w1);
+#else
+    const uint64_t h = (((uint64_t)a1 * w0) >> 32) + (((uint64_t)a0 * w1) >> 32);
+#endif
+    const uint64_t q = (uint64_t)a1 * w1 + h;
+    return a * w - q * Prime<PI>::P;
+}
+
 /* Shoup product a*w mod p for a < 2^32; result in [0, 2p).  q = hi64(a*wp)
  * exactly: a*wp = a*wp1*2^32 + a*wp0, and with t = a*wp1 + hi32(a*wp0) (an
  * integer below 2^64) floor(a*wp / 2^64) = floor(t / 2^32), since the

@@ -2,7 +2,7 @@
  * Host check of the modular primitives in cryptography_amd/csrc/modarith.h
  * against 128-bit integer arithmetic, for the three NTT primes: the value and
  * the documented output range of the Montgomery product, the Shoup product
- * (and its small-operand form), the one-step reduction of any 64-bit value, the
+ * (its small-operand form and its truncated-quotient form), the one-step reduction of any 64-bit value, the
  * compile-time roots, the lazy butterfly groups of the transform rounds, and
  * the CRT tests that three residues are those of one
  * integer below p0, or below 2 p0 (against Garner's digits).
@@ -59,6 +59,12 @@ static void check(long iters)
         const uint64_t r = shoup_lazy<PI>(a, w, wp);
         expect(r < 2 * p && r % p == (uint64_t)((u128)a * w % p), "shoup", PI, a, w);
     };
+    /* Shoup with the truncated quotient: any a, result below 4p */
+    auto shoup4 = [&](uint64_t a, uint64_t w) {
+        const uint64_t wp = (uint64_t)(((u128)w << 64) / p);
+        const uint64_t r = shoup_lazy4<PI>(a, w, wp);
+        expect(r < 4 * p && r % p == (uint64_t)((u128)a * w % p), "shoup4", PI, a, w);
+    };
     /* small-operand Shoup: the same quotient as the full one, so the same value */
     auto shoup_s = [&](uint32_t a, uint64_t w) {
         const uint64_t wp = (uint64_t)(((u128)w << 64) / p);
@@ -82,6 +88,7 @@ static void check(long iters)
         for (uint64_t b : edges) {
             if (b < 2 * p) mont(a, b);
             if (b < p) shoup(a, b);
+            if (b < p) shoup4(a, b);
             if (b < p && a <= 0xffffffffull) shoup_s((uint32_t)a, b);
         }
     for (long i = 0; i < iters; i++) {
@@ -92,6 +99,9 @@ static void check(long iters)
         mont(rnd() % (2 * p), rnd() % (2 * p));
         mont(rnd() % (4 * p), rnd() % p);
         shoup(rnd(), rnd() % p);
+        shoup4(rnd(), rnd() % p);
+        shoup4(~0ull - (rnd() & 0xffffff), p - 1 - (rnd() & 0xffff));
+        shoup4(rnd() | 0xffffffff00000000ull, rnd() % p);
         shoup_s((uint32_t)rnd(), rnd() % p);
         shoup_s((uint32_t)(rnd() & 0xffff), rnd() % p);
     }
