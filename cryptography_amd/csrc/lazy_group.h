@@ -17,9 +17,12 @@ namespace nttc {
  * (16p < 2^64 as p < 2^60):
  *   - forward (DIF) butterfly (x, y) -> (x + y, (x - y + k p) * w), k the
  *     bound of y: the sum's bound is the sum of both bounds, the Shoup
- *     product is below 2p for any input;
+ *     product is below 2p for any input (below 4p in a loose step, which
+ *     uses the truncated quotient of shoup_lazy4: one multiply, a carry and
+ *     a register move fewer);
  *   - inverse (DIT) butterfly (x, y) -> (x + t, x - t + 2p), t = y*w below
- *     2p for any y: both bounds are x's plus 2;
+ *     2p for any y: both bounds are x's plus 2 (in a loose step t is below
+ *     4p, the difference adds 4p, and both bounds are x's plus 4);
  *   - a butterfly with twiddle 1 keeps (x + y, x - y + k p): both below the
  *     sum of the bounds.
  * Where a butterfly would pass LZ_MAX its larger input is reduced first
@@ -57,7 +60,7 @@ __host__ __device__ constexpr LzPlan lz_plan_mask(int RL, bool dit, bool tw1, in
     const int R = 1 << RL;
     for (int i = 0; i < R; i++) b[i] = bin;
     P.ok = bin <= LZ_MAX && bout >= 2 && RL >= 1 && RL <= 3;
-    for (int j = 0; j < 3; j++) P.loose[j] = !dit && j < RL && ((loosemask >> j) & 1);
+    for (int j = 0; j < 3; j++) P.loose[j] = j < RL && ((loosemask >> j) & 1);
     for (int j = j0; j < RL; j++) {
         const int S = dit ? RL - 1 - j : j;
         const int half = R >> (S + 1);
@@ -66,13 +69,15 @@ __host__ __device__ constexpr LzPlan lz_plan_mask(int RL, bool dit, bool tw1, in
             const int y = i + half;
             const bool one = tw1 && (i & (half - 1)) == 0;
             if (dit && !one) {
-                if (b[i] + 2 > LZ_MAX) {
+                const int sb = P.loose[j] ? 4 : 2;     /* t = y*w below sb*p */
+                if (b[i] + sb > LZ_MAX) {
                     P.pre[j][i] = true;
                     P.cost += 5;
                     b[i] = 2;
                 }
-                P.k[j][i] = 2;
-                b[i] = b[y] = b[i] + 2;
+                P.k[j][i] = (unsigned char)sb;
+                b[i] = b[y] = b[i] + sb;
+                if (P.loose[j]) P.cost -= 4;
             } else {
                 for (int t = 0; t < 2; t++) {
                     if (b[i] + b[y] > LZ_MAX) {
@@ -102,22 +107,28 @@ __host__ __device__ constexpr LzPlan lz_plan_mask(int RL, bool dit, bool tw1, in
     return P;
 }
 
-/* The plan of least cost over the choices of loose forward steps.  A loose
- * step's products (below 4p) feed later butterflies of the group, which take
- * values up to 16p; where the group's outputs must leave below 2p (LDS and
- * HBM), loose products in the last step would each cost a reduction, so the
- * cost decides.  Inverse steps stay exact: an inverse product t enters both
- * outputs, x + t and x - t + k p, so a loose one raises every bound
- * downstream.  Ties keep fewer loose steps. */
+/* The plan of least cost over the choices of loose steps (those whose
+ * products use shoup_lazy4, below 4p instead of 2p).  A loose forward
+ * step's products feed later butterflies of the group, which take values up
+ * to 16p; where the group's outputs must leave below 2p (LDS and HBM), loose
+ * products in the last step would each cost a reduction, so the cost decides.
+ * A loose inverse product t enters both outputs, x + t and x - t + 4p, so it
+ * raises both bounds by 4 instead of 2.  Ties keep fewer loose steps, then
+ * the lower output bound (the next inverse round's input bound). */
 __host__ __device__ constexpr LzPlan lz_plan(int RL, bool dit, bool tw1, int bin, int bout, int j0)
 {
     LzPlan best = lz_plan_mask(RL, dit, tw1, bin, bout, j0, 0);
 #if NTT_LOOSE_SHOUP
-    if (!dit) {
-        for (unsigned m = 1; m < (1u << RL); m++) {
-            if (m & ((1u << j0) - 1)) continue;
-            const LzPlan c = lz_plan_mask(RL, dit, tw1, bin, bout, j0, m);
-            if (c.ok && c.cost < best.cost) best = c;
+    int bmax = 0;
+    for (int i = 0; i < (1 << RL); i++) bmax = best.out[i] > bmax ? best.out[i] : bmax;
+    for (unsigned m = 1; m < (1u << RL); m++) {
+        if (m & ((1u << j0) - 1)) continue;
+        const LzPlan c = lz_plan_mask(RL, dit, tw1, bin, bout, j0, m);
+        int cmax = 0;
+        for (int i = 0; i < (1 << RL); i++) cmax = c.out[i] > cmax ? c.out[i] : cmax;
+        if (c.ok && (c.cost < best.cost || (c.cost == best.cost && cmax < bmax))) {
+            best = c;
+            bmax = cmax;
         }
     }
 #endif
@@ -159,7 +170,9 @@ __host__ __device__ __forceinline__ void lz_step(uint64_t *x, const TW &tw)
                     else x[Y] = shoup_lazy<PI>(a - b + KP, w.w, w.wp);
                 } else {
                     const TwPair w = tw.template get<S, I>();
-                    const uint64_t t = shoup_lazy<PI>(b, w.w, w.wp);
+                    uint64_t t;
+                    if constexpr (RD::plan.loose[J]) t = shoup_lazy4<PI>(b, w.w, w.wp);
+                    else t = shoup_lazy<PI>(b, w.w, w.wp);
                     if constexpr (TW::NEG) {     /* w = -(the twiddle): sum and difference trade places */
                         x[I] = a - t + KP;
                         x[Y] = a + t;
