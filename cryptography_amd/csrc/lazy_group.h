@@ -33,9 +33,6 @@ namespace nttc {
  * of 12.  The plan of a group is computed by the compiler (lz_plan) and
  * every reduction and multiple of p in the code is a compile-time choice. */
 #define LZ_MAX 16
-#ifndef NTT_LOOSE_SHOUP
-#define NTT_LOOSE_SHOUP 1
-#endif
 
 struct LzPlan {
     unsigned char k[3][8];     /* step j, x-side index i: the multiple of p in the difference */
@@ -113,12 +110,11 @@ __host__ __device__ constexpr LzPlan lz_plan_mask(int RL, bool dit, bool tw1, in
  * to 16p; where the group's outputs must leave below 2p (LDS and HBM), loose
  * products in the last step would each cost a reduction, so the cost decides.
  * A loose inverse product t enters both outputs, x + t and x - t + 4p, so it
- * raises both bounds by 4 instead of 2.  Ties keep fewer loose steps, then
- * the lower output bound (the next inverse round's input bound). */
+ * raises both bounds by 4 instead of 2.  Of plans of equal cost the one
+ * with the lower output bound (the next inverse round's input bound) wins. */
 __host__ __device__ constexpr LzPlan lz_plan(int RL, bool dit, bool tw1, int bin, int bout, int j0)
 {
     LzPlan best = lz_plan_mask(RL, dit, tw1, bin, bout, j0, 0);
-#if NTT_LOOSE_SHOUP
     int bmax = 0;
     for (int i = 0; i < (1 << RL); i++) bmax = best.out[i] > bmax ? best.out[i] : bmax;
     for (unsigned m = 1; m < (1u << RL); m++) {
@@ -131,7 +127,6 @@ __host__ __device__ constexpr LzPlan lz_plan(int RL, bool dit, bool tw1, int bin
             bmax = cmax;
         }
     }
-#endif
     return best;
 }
 
