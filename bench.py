@@ -123,7 +123,8 @@ def launch_table(L, logL, na, nb, square=False):
 
     out, done = [], 0
     for i, s in enumerate(plan[:-1]):
-        kind = 3 if i == 0 else (6 if has_xtab(s, logL - done - s) else 0)
+        # the first pass takes its chain ratios from a table at S = 2^6 and 2^9 (kind 8)
+        kind = (8 if s in (6, 9) else 3) if i == 0 else (6 if has_xtab(s, logL - done - s) else 0)
         byt = 8 * (na + (0 if square else nb)) + nops * K_PRIMES * 8 * Lw if i == 0 else nops * K_PRIMES * 16 * Lw
         out.append((f"k_fast_pass<{s}, {kind}>", blocks(kind) * K_PRIMES * nops, byt))
         done += s
@@ -134,8 +135,8 @@ def launch_table(L, logL, na, nb, square=False):
         logT = logL - done - plan[i]
         logC = min(logT, 12 - plan[i])
         kind = 5 if plan[i] >= 3 and logC + plan[i] == 12 else 1
-        if kind == 5 and has_xtab(plan[i], logT):
-            kind = 7
+        if kind == 5:   # cross twiddles from a table (7), or chain ratios from one (9)
+            kind = 7 if has_xtab(plan[i], logT) else 9
         out.append((f"k_fast_pass<{plan[i]}, {kind}>", blocks(kind) * K_PRIMES, K_PRIMES * 16 * Lw))
     out.append(("k_garner_pack", None, K_PRIMES * 8 * Lw + 8 * (na + nb)))
     return out

@@ -53,6 +53,10 @@ struct Engine {
      * 2^NTT_XTAB_MAX_LOG residues, by (log2 S, log2 S*T), made on first use;
      * use_xtab = 0 keeps every pass on the in-round Montgomery chains */
     uint64_t *xtab[FT_MAX_LOGS + 1][NTT_XTAB_MAX_LOG + 1] = {};
+    /* chain ratio tables (B per column with its Shoup partner) of passes of
+     * 2^logS rows and 2^logN residues per block, made on first use; they
+     * follow use_xtab */
+    TwPair *btab[FT_MAX_LOGS + 1][NTT_MAX_LOG_LEN + 1] = {};
     int use_xtab = 1;
     /* set while a call that must not allocate runs: xtab_for then hands out
      * only the tables that already exist (the passes fall back to chains) */
@@ -169,6 +173,7 @@ void class_needs(int logL, uint32_t nprod, uint32_t max_nout, int square, size_t
 /* Cross-twiddle table pair (forward, inverse) of an S x T pass, or nullptr
  * (tables off, block too large, or not yet made while a stream captures). */
 const uint64_t *xtab_for(int logS, int logT, hipStream_t st);
+const TwPair *btab_for(int logS, int logT, hipStream_t st);
 /* Make the tables the passes of a length-2^logL transform use (before a
  * graph capture, which cannot allocate). */
 void xtab_prepare(int logL);

@@ -160,6 +160,9 @@ void engine_teardown()
     for (auto &row : g.xtab)
         for (uint64_t *t : row)
             if (t) (void)hipFree(t);
+    for (auto &row : g.btab)
+        for (TwPair *t : row)
+            if (t) (void)hipFree(t);
     g_ws.release();
     if (g.d_desc) (void)hipFree(g.d_desc);
     if (g.d_a) (void)hipFree(g.d_a);
@@ -271,6 +274,36 @@ const uint64_t *xtab_for(int logS, int logT, hipStream_t st)
     return t;
 }
 
+/* Chain ratio tables (launch_btab_gen): the first forward pass of 8-point
+ * last-round groups (kind 8) and the inverse passes that apply their cross
+ * twiddles at the load without a table of them (kind 9) step their chains by
+ * Shoup products with B from here instead of Montgomery products with B
+ * looked up per thread.  2 x 3 x T entries of 16 bytes (24 MB for the first
+ * pass at L = 2^27); nullptr where the passes compute B themselves. */
+const TwPair *btab_for(int logS, int logT, hipStream_t st)
+{
+    const int logN = logS + logT;
+    if (!g.use_xtab || logS < 3 || logS > FT_MAX_LOGS || logT < 0 || logN > NTT_MAX_LOG_LEN) return nullptr;
+    TwPair *&t = g.btab[logS][logN];
+    if (t) return t;
+    if (g.no_alloc) return nullptr;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone, cg = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return nullptr;
+    if (hipStreamIsCapturing(g.stream, &cg) != hipSuccess || cg != hipStreamCaptureStatusNone) return nullptr;
+    TwPair *p = nullptr;
+    if (hipMalloc((void **)&p, (size_t)NTT_BTAB_DIRS * NTT_NPRIMES * sizeof(TwPair) << logT) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (launch_btab_gen(g.tw, p, logS, logT, g.stream) != hipSuccess || hipStreamSynchronize(g.stream) != hipSuccess) {
+        (void)hipGetLastError();
+        (void)hipFree(p);
+        return nullptr;
+    }
+    t = p;
+    return t;
+}
+
 void xtab_prepare(int logL)
 {
     if (logL <= 0) return;
@@ -278,6 +311,7 @@ void xtab_prepare(int logL)
     int done = 0;
     for (int i = 0; i < pp.npass - 1; i++) {
         (void)xtab_for((int)pp.logS[i], logL - done - (int)pp.logS[i], g.stream);
+        (void)btab_for((int)pp.logS[i], logL - done - (int)pp.logS[i], g.stream);
         done += (int)pp.logS[i];
     }
 }
@@ -423,7 +457,11 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         pa.half_zero = pa.first && max_nab && (uint64_t)8 * max_nab <= L;
         int kind = pa.first ? 3 : 0;
         pa.xtab = nullptr;
+        pa.btab = nullptr;
         if (kind == 0 && (pa.xtab = xtab_for((int)pp.logS[i], (int)pa.logT, st))) kind = 6;
+        if (kind == 3 && (pp.logS[i] == 6 || pp.logS[i] == 9) &&
+            (pa.btab = btab_for((int)pp.logS[i], (int)pa.logT, st)))
+            kind = 8;      /* dir 0 of the table */
         pa.tpb = tiles_per_block(ntiles, kind);
         HIP_TRY(launch_fast_pass(pp.logS[i], kind, pa, ntiles, fwd_ops, st));
         HIP_TRY(mark("k_fast_pass<fwd>"));
@@ -446,6 +484,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     const bool scale_late = pp.npass >= 2 && pp.logS[0] >= 3 && pa.logC + pp.logS[0] == 12;
     geom(pp.npass - 1, done);
     pa.xtab = nullptr;
+    pa.btab = nullptr;
     pa.half_zero = 0;
     pa.tpb = tiles_per_block(ntiles, 0);
     pa.first = pp.npass == 1 && !residues_in;
@@ -465,12 +504,17 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
         int kind = (pp.logS[i] >= 3 && pa.logC + pp.logS[i] == 12) ? 5 : 1;
         pa.xtab = nullptr;
         pa.iscale = i == 0 && scale_late;
+        pa.btab = nullptr;
         if (kind == 5) {
             const uint64_t *xt = xtab_for((int)pp.logS[i], (int)pa.logT, st);
+            const TwPair *bt = xt ? nullptr : btab_for((int)pp.logS[i], (int)pa.logT, st);
             if (xt) {   /* inverse tables follow the forward one; pass 0's block is
                            the whole transform, so its scaled table is the 1/L one */
                 kind = 7;
                 pa.xtab = xt + ((size_t)(pa.iscale ? 2 : 1) * NTT_NPRIMES << (pp.logS[i] + pa.logT));
+            } else if (bt) {   /* dir 1: B = w_N^-r */
+                kind = 9;
+                pa.btab = bt + ((size_t)NTT_NPRIMES << pa.logT);
             }
         }
         HIP_TRY(launch_fast_pass(pp.logS[i], kind, pa, ntiles, nops, st));

@@ -184,7 +184,10 @@ struct XtwCtx {
     uint32_t col0;        /* first column of the tile */
     uint32_t shift;       /* NTT_ROOT_LOG - log2(N): exponents of the 2^30 root */
     uint64_t ta, tb;      /* A, B of this thread's group (Montgomery form) when
-                             the caller looked them up ahead (xt_ahead) */
+                             the caller looked them up ahead (xt_ahead); with a
+                             ratio table (kind 8), tb is B itself, in [0, p) */
+    uint64_t tbp;         /* kind 8: B's Shoup partner */
+    const TwPair *btab;   /* kind 8: this prime's ratios, B of column r at btab[r] */
 };
 
 /* When the cross-twiddle round has 8-point groups and one group per thread
@@ -249,6 +252,21 @@ __device__ __forceinline__ void chain_apply(uint64_t *x, uint64_t t, const uint6
         if (k + 1 < R) t = mont_mul_lazy<PI>(t, b);
     }
 }
+/* The same with the ratio B in plain form with its Shoup partner (a ratio
+ * table): the chain's steps are Shoup products (14 VALU instructions against
+ * the Montgomery product's 24), which keep t in Montgomery form and below
+ * 2p, as the products by t need. */
+template <int PI, int RL>
+__device__ __forceinline__ void chain_apply_tab(uint64_t *x, uint64_t t, const uint64_t b, const uint64_t bp)
+{
+    constexpr int R = 1 << RL;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int i = RL == 3 ? ((k & 1) << 2) | (k & 2) | ((k >> 2) & 1) : RL == 2 ? ((k & 1) << 1) | (k >> 1) : k;
+        x[i] = mont_mul_lazy<PI>(x[i], t);
+        if (k + 1 < R) t = shoup_lazy<PI>(t, b, bp);
+    }
+}
 template <int PI, int RL>
 __device__ __forceinline__ void chain_twiddle(uint64_t *x, const TwPair *tw, uint32_t EA, uint32_t EB)
 {
@@ -260,7 +278,7 @@ __device__ __forceinline__ void chain_twiddle(uint64_t *x, const TwPair *tw, uin
  * column col0 (xt_ahead): the group is thread tid's, rows m0 .. m0+7 with
  * m0 = ((tid >> logC) mod 2^(LOGS-3)) * 8, column r = col0 + (tid mod C);
  * INV gives the inverse twiddles. */
-template <int PI, int LOGS, bool INV>
+template <int PI, int LOGS, bool INV, bool BTAB = false>
 __device__ __forceinline__ void xt_lookup(XtwCtx &xc, uint32_t logC)
 {
     uint32_t tid = threadIdx.x;
@@ -273,7 +291,14 @@ __device__ __forceinline__ void xt_lookup(XtwCtx &xc, uint32_t logC)
         eb = 0u - eb;
     }
     xc.ta = tw_mont<PI>(xc.tw, ea & ((1u << NTT_ROOT_LOG) - 1));
-    xc.tb = tw_mont<PI>(xc.tw, eb & ((1u << NTT_ROOT_LOG) - 1));
+    if constexpr (BTAB) {
+        static_assert(!INV, "ratio tables serve the first forward pass");
+        const TwPair b = xc.btab[r];
+        xc.tb = b.w;
+        xc.tbp = b.wp;
+    } else {
+        xc.tb = tw_mont<PI>(xc.tw, eb & ((1u << NTT_ROOT_LOG) - 1));
+    }
 }
 
 /* ---- stage twiddles of the fused passes: one table, both directions --------
@@ -484,10 +509,11 @@ __device__ __forceinline__ uint32_t last_group(const PassArgs &pa, const FGeom &
  * group), stored from registers: no LDS round trip and no block barrier
  * before the tile leaves (measured on the first pass of a 2^30-bit product:
  * 3.89 -> 3.43 ms, and 78 -> 70 VGPRs). */
-template <int PI, int LOGS>
+template <int PI, int LOGS, bool BTAB = false>
 __device__ __forceinline__ void dif_last_store(const PassArgs &pa, const FGeom &g, uint32_t tt, const uint64_t *lds,
                                                uint64_t *x, const XtwCtx &xc)
 {
+    static_assert(!BTAB || (last_rl<LOGS>() == 3 && xt_ahead<LOGS>()), "ratio tables: 8-point groups, B ahead");
     constexpr int RL = last_rl<LOGS>(), R = 1 << RL, NG = 8 >> RL;
     uint32_t tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
@@ -500,7 +526,9 @@ __device__ __forceinline__ void dif_last_store(const PassArgs &pa, const FGeom &
 #pragma unroll
         for (int i = 0; i < R; i++) v[i] = lds[LP(q0 + ((uint32_t)i << g.logC))];
         const_dif<PI, RL, 8>(v);     /* the chain takes values below 8p to (0, 2p) */
-        if constexpr (RL == 3 && xt_ahead<LOGS>()) {
+        if constexpr (BTAB) {
+            chain_apply_tab<PI, 3>(v, xc.ta, xc.tb, xc.tbp);
+        } else if constexpr (RL == 3 && xt_ahead<LOGS>()) {
             chain_apply<PI, 3>(v, xc.ta, xc.tb);
         } else {
             const uint32_t r = xc.col0 + c;
@@ -713,21 +741,25 @@ __device__ __forceinline__ void load_tile(const PassArgs &pa, const uint64_t *x,
  * col0 + (q mod C)): one Montgomery product per residue instead of the
  * in-round chain's two, for passes whose S x T block is at most
  * 2^NTT_XTAB_MAX_LOG residues (its table stays in the L2 caches). */
-template <int PI, int LOGS, bool FIRST, bool XTAB>
+/* BTAB (the first pass, 8-point last-round groups): the chain's ratio B
+ * comes from pa.btab with its Shoup partner. */
+template <int PI, int LOGS, bool FIRST, bool XTAB, bool BTAB = false>
 __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds, TwPair *twf)
 {
+    static_assert(!BTAB || (FIRST && LOGS >= 4 && xt_ahead<LOGS>()), "ratio tables: first pass, B ahead");
     const FGeom g{pa.logT, pa.logC, pa.logG};
     uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
     const uint32_t colTiles = g.logT - g.logC;
     uint32_t tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;   /* the tile worked on */
-    XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0};
+    XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0, 0,
+              BTAB ? pa.btab + ((size_t)PI << g.logT) : nullptr};
     /* the first tile and the cross-twiddle look-ups are in flight while
      * the stage twiddles are written */
     stage_twiddles<PI, LOGS, 1>(pa.small_tw, twf, nullptr);
     if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, tt, lds);
     xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
-    if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false>(xc, g.logC);
+    if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false, BTAB>(xc, g.logC);
     if constexpr (!FIRST) WAIT_VMCNT(0);
     lds_sync();
     if constexpr (XTAB && !FIRST && LOGS >= 3) {
@@ -805,7 +837,7 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
             lds_sync();
             if constexpr (REGSTORE) {
                 dif_above_last<PI, LOGS, LOGS - 4>(lds, twf, g.logC);
-                dif_last_store<PI, LOGS>(pa, g, tt, lds, x, xc);
+                dif_last_store<PI, LOGS, BTAB>(pa, g, tt, lds, x, xc);
             } else {
                 dif_all<PI, LOGS, LOGS - 4, true>(lds, twf, g.logC, xc);
             }
@@ -826,7 +858,7 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
         lds_sync();
         if constexpr (!FIRST) tile_dma<LOGS>(pa, x, g, tt, lds);
         xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
-        if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false>(xc, g.logC);
+        if constexpr (xt_ahead<LOGS>() && !XTAB) xt_lookup<PI, LOGS, false, BTAB>(xc, g.logC);
         if constexpr (!FIRST) {
             WAIT_VMCNT(0);
             lds_sync();
@@ -840,10 +872,11 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
  * cross twiddles are applied as a chain right after the load and the DIT
  * rounds carry no twiddle code (fewer live registers).  Otherwise they are
  * applied in the first DIT round. */
-template <int PI, int LOGS, bool LOADCHAIN, bool XTAB>
+template <int PI, int LOGS, bool LOADCHAIN, bool XTAB, bool BTAB = false>
 __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair *twi)
 {
     static_assert(LOADCHAIN || !XTAB, "table twiddles are applied at the load");
+    static_assert(!BTAB || (LOADCHAIN && !XTAB), "ratio tables serve the load chain");
     const FGeom g{pa.logT, pa.logC, pa.logG};
     uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
@@ -879,7 +912,13 @@ __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair
                 constexpr uint32_t MASK = (1u << NTT_ROOT_LOG) - 1;
                 uint64_t ta = tw_mont<PI>(xc.tw, (0u - ((r * brev_n(m0, LOGS)) << xc.shift)) & MASK);
                 if (pa.iscale) ta = mont_mul_lazy<PI>(ta, pa.scale_m[PI]);   /* the 1/L rides on the chain */
-                chain_apply<PI, 3>(v, ta, tw_mont<PI>(xc.tw, (0u - (r << xc.shift)) & MASK));
+                if constexpr (BTAB) {
+                    /* B = w_N^-r and its Shoup partner from the ratio table */
+                    const TwPair b = pa.btab[((size_t)PI << g.logT) + r];
+                    chain_apply_tab<PI, 3>(v, ta, b.w, b.wp);
+                } else {
+                    chain_apply<PI, 3>(v, ta, tw_mont<PI>(xc.tw, (0u - (r << xc.shift)) & MASK));
+                }
             }
 #pragma unroll
             for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
@@ -1100,7 +1139,9 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
 /* kind 0: forward pass, 1: inverse pass, 2: fused middle pass,
  * 3: first forward pass (reads the operands), 4: single fused pass,
  * 5: inverse pass with the cross twiddles applied at load (C * S = 4096),
- * 6: kind 0 and 7: kind 5 with the cross twiddles from pa.xtab. */
+ * 6: kind 0 and 7: kind 5 with the cross twiddles from pa.xtab,
+ * 8: kind 3 (S = 2^6, 2^9) and 9: kind 5 with the chains' ratios from
+ * pa.btab. */
 template <int PI, int LOGS, int KIND>
 __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, int op,
                                           uint64_t *lds, TwPair *tw)
@@ -1113,6 +1154,8 @@ __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, cons
     else if constexpr (KIND == 1) inv_tiles<PI, LOGS, false, false>(pa, xa, lds, tw);
     else if constexpr (KIND == 5) inv_tiles<PI, LOGS, true, false>(pa, xa, lds, tw);
     else if constexpr (KIND == 7) inv_tiles<PI, LOGS, true, true>(pa, xa, lds, tw);
+    else if constexpr (KIND == 8) fwd_tiles<PI, LOGS, true, false, true>(pa, xa, op, lds, tw);
+    else if constexpr (KIND == 9) inv_tiles<PI, LOGS, true, false, true>(pa, xa, lds, tw);
     else if constexpr (KIND == 2) mid_tile<PI, LOGS, false>(pa, xa, xb, lds, tw);
     else mid_tile<PI, LOGS, true>(pa, xa, xb, lds, tw);
 }
@@ -1130,10 +1173,13 @@ constexpr uint32_t tw_lds_entries()
 /* The fused middle pass of 512 points (kind 2, S = 2^9: the passes of the
  * longest products) asks for 8 waves per SIMD: at 40 KB of LDS four blocks
  * fit a CU, and the register allocation keeps it at 64 VGPRs without
- * spilling.  Everything else keeps its allocation (capped at 64 VGPRs the
- * other kinds, and the fused pass at other sizes, spill). */
+ * spilling.  So does the 512-point inverse pass with a ratio table (kind 9:
+ * 60 VGPRs, 66 by default, where kind 5 takes 60).  Everything else keeps
+ * its allocation (capped at 64 VGPRs the other kinds, and the fused pass at
+ * other sizes, spill). */
 template <int LOGS, int KIND>
-__global__ void __launch_bounds__(FTPB) __attribute__((amdgpu_waves_per_eu(KIND == 2 && LOGS == 9 ? 8 : 1)))
+__global__ void __launch_bounds__(FTPB)
+    __attribute__((amdgpu_waves_per_eu((KIND == 2 || KIND == 9) && LOGS == 9 ? 8 : 1)))
 k_fast_pass(PassArgs pa)
 {
     __shared__ uint64_t lds[FT_LDS];
@@ -1153,15 +1199,20 @@ k_fast_pass(PassArgs pa)
 hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntiles, int nops, hipStream_t st)
 {
     /* forward and inverse passes: tpb tiles per block; fused passes: one */
-    const bool looped = kind == 0 || kind == 1 || kind == 3 || kind == 5 || kind == 6 || kind == 7;
+    const bool looped = kind == 0 || kind == 1 || kind == 3 || kind == 5 || kind == 6 || kind == 7 || kind == 8 ||
+                        kind == 9;
     if (looped && (pa.tpb < 1 || pa.ntiles != ntiles)) return hipErrorInvalidValue;
-    if ((kind == 5 || kind == 7) && (logS < 3 || pa.logC + logS != 12)) return hipErrorInvalidValue;
+    if ((kind == 5 || kind == 7 || kind == 9) && (logS < 3 || pa.logC + logS != 12)) return hipErrorInvalidValue;
+    /* ratio tables: the first pass with 8-point last-round groups, and the
+     * load chain */
+    if ((kind == 8 || kind == 9) && !pa.btab) return hipErrorInvalidValue;
+    if (kind == 8 && (!pa.first || (logS != 6 && logS != 9))) return hipErrorInvalidValue;
     /* tile transfers move column pairs (C >= 2) or row pairs (T = 1) */
     if ((kind == 0 || kind == 1 || kind == 6) && pa.logC == 0 && pa.logT != 0) return hipErrorInvalidValue;
     /* table twiddles: a table of this pass's S x T block */
     if ((kind == 6 || kind == 7) && (!pa.xtab || logS + (int)pa.logT > NTT_XTAB_MAX_LOG)) return hipErrorInvalidValue;
     const size_t nblk = looped ? (ntiles + pa.tpb - 1) / pa.tpb : ntiles;
-    const bool fwd = kind == 0 || kind == 3 || kind == 6;
+    const bool fwd = kind == 0 || kind == 3 || kind == 6 || kind == 8;
     dim3 grid((unsigned)nblk, NTT_NPRIMES, fwd ? nops : 1);
     dim3 block(FTPB);
 #define L1(S)                                                                                  \
@@ -1186,9 +1237,15 @@ hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntile
         case 5: hipLaunchKernelGGL((k_fast_pass<S, 5>), grid, block, 0, st, pa); break;        \
         case 6: hipLaunchKernelGGL((k_fast_pass<S, 6>), grid, block, 0, st, pa); break;        \
         case 7: hipLaunchKernelGGL((k_fast_pass<S, 7>), grid, block, 0, st, pa); break;        \
+        case 9: hipLaunchKernelGGL((k_fast_pass<S, 9>), grid, block, 0, st, pa); break;        \
         default: return hipErrorInvalidValue;                                                  \
         }                                                                                      \
         break;
+    if (kind == 8) {   /* checked above: S = 2^6 or 2^9 */
+        if (logS == 6) hipLaunchKernelGGL((k_fast_pass<6, 8>), grid, block, 0, st, pa);
+        else hipLaunchKernelGGL((k_fast_pass<9, 8>), grid, block, 0, st, pa);
+        return hipGetLastError();
+    }
     switch (logS) {
         L1(1) L1(2) L5(3) L5(4) L5(5) L5(6) L5(7) L5(8) L5(9) L5(10)
     default:
@@ -1205,6 +1262,46 @@ hipError_t launch_xtab_gen(const TwPair *tw, uint64_t *out, int logS, int logT, 
     if (logS < 1 || logT < 0 || logN > NTT_XTAB_MAX_LOG || logN < 8) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_xtab_gen, dim3(1u << (logN - 8), NTT_NPRIMES, NTT_XTAB_DIRS), dim3(256), 0, st, tw, out,
                        (uint32_t)logS, (uint32_t)logT, sc);
+    return hipGetLastError();
+}
+
+/* grid (T / 256, primes, NTT_BTAB_DIRS): the chain ratios (layout in ntt_kernels.h) */
+template <int PI>
+__device__ __forceinline__ TwPair btab_entry(const TwPair *tw, uint32_t logS, uint32_t logT, uint32_t r, bool inv)
+{
+    const uint32_t shift = NTT_ROOT_LOG - logS - logT;
+    const uint32_t E = (inv ? 0u - (r << shift) : (r << (logS - 3)) << shift) & ((1u << NTT_ROOT_LOG) - 1);
+    const TwPair lo = tw[E & (NTT_TW_SIZE - 1)], hi = tw[NTT_TW_SIZE + (E >> NTT_TW_LOG)];
+    const uint64_t w = reduce1<PI>(shoup_lazy<PI>(lo.w, hi.w, hi.wp));
+    /* floor(w * 2^64 / p) bit by bit (w < p < 2^60: no overflow) */
+    uint64_t rem = w, q = 0;
+    for (int i = 63; i >= 0; i--) {
+        rem <<= 1;
+        if (rem >= Prime<PI>::P) {
+            rem -= Prime<PI>::P;
+            q |= (uint64_t)1 << i;
+        }
+    }
+    return TwPair{w, q};
+}
+__global__ void __launch_bounds__(256) k_btab_gen(const TwPair *tw, TwPair *out, uint32_t logS, uint32_t logT)
+{
+    const uint32_t T = 1u << logT;
+    const uint32_t r = blockIdx.x * 256 + threadIdx.x, pi = blockIdx.y;
+    const bool inv = blockIdx.z != 0;
+    if (r >= T) return;
+    const TwPair *t = tw + (size_t)pi * 2 * NTT_TW_SIZE;
+    out[((size_t)blockIdx.z * NTT_NPRIMES + pi) * T + r] = pi == 0   ? btab_entry<0>(t, logS, logT, r, inv)
+                                                           : pi == 1 ? btab_entry<1>(t, logS, logT, r, inv)
+                                                                     : btab_entry<2>(t, logS, logT, r, inv);
+}
+
+hipError_t launch_btab_gen(const TwPair *tw, TwPair *out, int logS, int logT, hipStream_t st)
+{
+    if (logS < 3 || logS > FT_MAX_LOGS || logT < 0 || logS + logT > NTT_MAX_LOG_LEN) return hipErrorInvalidValue;
+    const unsigned T = 1u << logT;
+    hipLaunchKernelGGL(k_btab_gen, dim3((T + 255) / 256, NTT_NPRIMES, NTT_BTAB_DIRS), dim3(256), 0, st, tw, out,
+                       (uint32_t)logS, (uint32_t)logT);
     return hipGetLastError();
 }
 

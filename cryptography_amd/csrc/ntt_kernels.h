@@ -68,6 +68,9 @@ struct PassArgs {
                                    kind 7 reads a table that includes it */
     const uint64_t *xtab;       /* kinds 6 and 7: this pass's cross twiddles, one
                                    entry per residue of an S x T block, per prime */
+    const TwPair *btab;         /* kinds 8 and 9: the cross-twiddle chain's ratio B
+                                   per column with its Shoup partner, T entries per
+                                   prime (one direction of a btab_for table) */
     const uint64_t *bres;       /* fused pass: b's rows (after all forward passes
                                    but the last) kept outside the class, one
                                    product long, prime i at bres + i*blen; every
@@ -91,6 +94,13 @@ struct XtabScale {
     uint64_t s[3], sp[3];       /* 2^64 / N mod p and Shoup partners */
 };
 hipError_t launch_xtab_gen(const TwPair *tw, uint64_t *out, int logS, int logT, const XtabScale &sc, hipStream_t st);
+/* Chain ratios of a pass with S = 2^logS rows and T = 2^logT columns (N = S*T):
+ * out[dir][prime][r] = (B, floor(B * 2^64 / p)) for column r < T, B in [0, p):
+ * dir 0 B = w_N^(r * S/8) (the first forward pass's 8-point groups of
+ * consecutive rows), dir 1 B = w_N^-r (the inverse pass that applies its
+ * cross twiddles at the load, rows m0 + k*S/8). */
+#define NTT_BTAB_DIRS 2
+hipError_t launch_btab_gen(const TwPair *tw, TwPair *out, int logS, int logT, hipStream_t st);
 
 struct GarnerConsts {
     uint64_t inv01, inv01p;      /* p0^-1 mod p1 */
