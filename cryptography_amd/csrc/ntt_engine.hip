@@ -419,8 +419,17 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
      * column slice, and the second one halves the stage-twiddle staging
      * (measured 3.75 -> 3.62 ms at L = 2^27; the other kinds are slower with
      * two) */
+    /* The load-time inverse passes (kinds 5, 7, 9) start each next tile's
+     * transfer during the current tile's top round, and take tiles nblk
+     * apart, so neighbouring column tiles still run side by side: measured at
+     * L = 2^27 (ms, 1 / 2 / 4 / 8 tiles per block) kind 7 1.75 / 1.71 / 1.71 /
+     * 1.67, kind 9 1.92 / 1.87 / 1.87 / 1.88 (1.78 and 1.94 with the
+     * register-loaded tiles before). */
     auto tiles_per_block = [&](size_t nt, int kind) {
-        const size_t cap = kind == 6 ? std::max(g.tile_batch, 2) : g.tile_batch;
+        const size_t cap = kind == 6   ? std::max(g.tile_batch, 2)
+                           : kind == 7 ? std::max(g.tile_batch, 8)
+                           : (kind == 5 || kind == 9) ? std::max(g.tile_batch, 4)
+                                                      : g.tile_batch;
         return (uint32_t)std::min<size_t>(std::max<size_t>(nt * NTT_NPRIMES / 4096, 1), cap);
     };
     pa.tpb = tiles_per_block(ntiles, 0);
@@ -517,6 +526,7 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
                 pa.btab = bt + ((size_t)NTT_NPRIMES << pa.logT);
             }
         }
+        pa.tpb = tiles_per_block(ntiles, kind);
         HIP_TRY(launch_fast_pass(pp.logS[i], kind, pa, ntiles, nops, st));
         pa.iscale = 0;
         HIP_TRY(mark("k_fast_pass<inv>"));

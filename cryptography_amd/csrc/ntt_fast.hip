@@ -543,15 +543,11 @@ __device__ __forceinline__ void dif_last_store(const PassArgs &pa, const FGeom &
  * stored from registers: thread tid's group is tile words tid + k*FTPB,
  * which tile_span places at gb + k*gs. */
 template <int PI, int LOGS, int BIN>
-__device__ __forceinline__ void dit_top_store(const uint64_t *lds, const TwPair *twi, uint32_t logC, uint64_t *x,
-                                              size_t gb, size_t gs)
+__device__ __forceinline__ void dit_top_finish(uint64_t *v, const TwPair *twi, uint32_t logC, uint64_t *x, size_t gb,
+                                               size_t gs)
 {
     constexpr int LH = LOGS - 1, RL = 3, R = 8, LD = LH - (RL - 1);
-    const uint32_t tid = threadIdx.x;
-    const uint32_t j = tid >> logC;            /* the group's first row, below S/8 = 2^LD */
-    uint64_t v[R];
-#pragma unroll
-    for (int i = 0; i < R; i++) v[i] = lds[LP(tid + i * FTPB)];
+    const uint32_t j = threadIdx.x >> logC;    /* the group's first row, below S/8 = 2^LD */
     /* inputs below BIN*p, stored unreduced, below 16p (inverse-pass output) */
     lz_group<PI, RL, true, false, BIN, LZ_MAX>(v, TabTwF<LH, LD, R, true, 0>{twi, j});
 #pragma unroll
@@ -711,30 +707,6 @@ __device__ __forceinline__ void tile_dma(const PassArgs &pa, const uint64_t *x, 
  * expcnt and lgkmcnt left at their maxima) */
 #define WAIT_VMCNT(N) __builtin_amdgcn_s_waitcnt(0x0F70 | ((N) & 15) | (((N) >> 4) << 14))
 
-/* Tiles of a forward or inverse pass: block b takes tiles
- * [b*tpb, min((b+1)*tpb, ntiles)) of its (operand, prime) row, b in XCD
- * order, one after the other (the stage twiddles are staged once per block).
- * Loading the next tile into registers during the current one's rounds was
- * measured slower: it costs the registers that let three blocks share a CU. */
-template <int LOGS, bool FIRST>
-__device__ __forceinline__ void load_tile(const PassArgs &pa, const uint64_t *x, int op, const FGeom &g, uint32_t t,
-                                          uint64_t *v)
-{
-    uint32_t tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));
-    if constexpr (FIRST) {
-#pragma unroll
-        for (int k = 0; k < FT_PER; k++) {
-            uint32_t q = tid + k * FTPB, m, r;
-            v[k] = first_in(pa, op, ft_addr(LOGS, g, t, q, m, r));
-        }
-    } else {
-        size_t base, step;
-        tile_span<LOGS>(pa, g, t, tid, base, step);
-#pragma unroll
-        for (int k = 0; k < FT_PER; k++) v[k] = x[base + k * step];
-    }
-}
 
 /* XTAB: the cross twiddles come from pa.xtab and multiply the tile as it is
  * stored (element q = tid + k*FTPB sits in row (q >> logC) mod S, column
@@ -866,95 +838,127 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
     }
 }
 
-/* LOADCHAIN (tiles of one block, C * S = 4096): thread tid loads rows
- * m0 + k * 2^(LOGS-3), k < 8, of column c (m0 = tid >> logC < 2^(LOGS-3)),
- * and bitrev(m0 + k * 2^(LOGS-3)) = bitrev(m0) + bitrev3(k), so the inverse
- * cross twiddles are applied as a chain right after the load and the DIT
- * rounds carry no twiddle code (fewer live registers).  Otherwise they are
- * applied in the first DIT round. */
-template <int PI, int LOGS, bool LOADCHAIN, bool XTAB, bool BTAB = false>
+/* The inverse passes that apply their cross twiddles at the load (kinds 5, 7
+ * and 9; C * S = 4096).  Thread tid's tile words tid + k*FTPB are rows
+ * m0 + k*2^(LOGS-3), k < 8, of column c (m0 = tid >> logC < 2^(LOGS-3)), and
+ * bitrev(m0 + k*2^(LOGS-3)) = bitrev(m0) + bitrev3(k), so their cross
+ * twiddles are one table entry each (kind 7) or a chain A*B^bitrev3(k)
+ * (kinds 5 and 9), applied in LDS before the rounds, which then carry no
+ * twiddle code.  Tiles reach LDS by DMA, and the next tile's transfer
+ * starts as soon as the top round has read its groups, so it is in flight
+ * during that round's butterflies and stores (the top round stores from
+ * registers).  A block takes tiles j, j + nblk, j + 2 nblk, ... (j its
+ * XCD-ordered index): neighbouring column tiles, which share 128-byte lines,
+ * still run side by side on one XCD. */
+template <int PI, int LOGS, bool XTAB, bool BTAB>
+__device__ void inv_tiles_dma(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair *twi)
+{
+    static_assert(!(XTAB && BTAB), "one source of cross twiddles");
+    const FGeom g{pa.logT, pa.logC, pa.logG};
+    const uint32_t nblk = gridDim.x;
+    uint32_t t = xcd_tile(blockIdx.x, nblk);
+    const uint32_t colTiles = g.logT - g.logC;
+    uint32_t tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;   /* the tile worked on */
+    XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0};
+    /* the first tile is in flight while the stage twiddles are written */
+    tile_dma<LOGS>(pa, x, g, tt, lds);
+    stage_twiddles<PI, LOGS, 2>(pa.small_tw, nullptr, twi);
+    for (;;) {
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
+        xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
+        const uint32_t r = xc.col0 + (tid & ((1u << g.logC) - 1)), m0 = tid >> g.logC;
+        uint64_t v[FT_PER];
+        if constexpr (XTAB) {
+            /* one table entry per word, loaded while the tile arrives */
+            const uint64_t *xt = pa.xtab + ((size_t)PI << (LOGS + g.logT)) + ((size_t)m0 << g.logT) + r;
+            uint64_t w[FT_PER];
+#pragma unroll
+            for (int k = 0; k < FT_PER; k++) w[k] = xt[(size_t)k << (LOGS - 3 + g.logT)];
+            WAIT_VMCNT(0);
+            lds_sync();
+#pragma unroll
+            for (int k = 0; k < FT_PER; k++) v[k] = mont_mul_lazy<PI>(lds[LP(tid + k * FTPB)], w[k]);
+        } else {
+            constexpr uint32_t MASK = (1u << NTT_ROOT_LOG) - 1;
+            uint64_t ta = tw_mont<PI>(xc.tw, (0u - ((r * brev_n(m0, LOGS)) << xc.shift)) & MASK);
+            if (pa.iscale) ta = mont_mul_lazy<PI>(ta, pa.scale_m[PI]);   /* the 1/L rides on the chain */
+            if constexpr (BTAB) {
+                /* B = w_N^-r and its Shoup partner from the ratio table */
+                const TwPair b = pa.btab[((size_t)PI << g.logT) + r];
+                WAIT_VMCNT(0);
+                lds_sync();
+#pragma unroll
+                for (int k = 0; k < FT_PER; k++) v[k] = lds[LP(tid + k * FTPB)];
+                chain_apply_tab<PI, 3>(v, ta, b.w, b.wp);
+            } else {
+                const uint64_t tb = tw_mont<PI>(xc.tw, (0u - (r << xc.shift)) & MASK);
+                WAIT_VMCNT(0);
+                lds_sync();
+#pragma unroll
+                for (int k = 0; k < FT_PER; k++) v[k] = lds[LP(tid + k * FTPB)];
+                chain_apply<PI, 3>(v, ta, tb);
+            }
+        }
+        /* each thread rewrites only the words it read */
+#pragma unroll
+        for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
+        lds_sync();
+        /* the table's Montgomery products (an entry below p) leave the tile
+         * below 2p, the chain's below 3p */
+        dit_all<PI, LOGS, LOGS - 4, false, XTAB>(lds, twi, g.logC, xc);
+        tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        size_t base, step;
+        tile_span<LOGS>(pa, g, tt, tid, base, step);
+#pragma unroll
+        for (int i = 0; i < FT_PER; i++) v[i] = lds[LP(tid + i * FTPB)];
+        t += nblk;
+        const bool more = t < pa.ntiles;
+        if (more) {
+            tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;
+            lds_sync();           /* every group read: the tile buffer is free */
+            tile_dma<LOGS>(pa, x, g, tt, lds);
+        }
+        dit_top_finish<PI, LOGS, dit_bound(LOGS - 4, XTAB ? 2 : 4)>(v, twi, g.logC, x, base, step);
+        if (!more) return;
+    }
+}
+
+/* Inverse passes without load-time cross twiddles (kind 1): the tile goes
+ * straight into LDS (in flight while the stage twiddles are written) and
+ * the inverse cross twiddles go into the first round, by chain. */
+template <int PI, int LOGS>
 __device__ void inv_tiles(const PassArgs &pa, uint64_t *x, uint64_t *lds, TwPair *twi)
 {
-    static_assert(LOADCHAIN || !XTAB, "table twiddles are applied at the load");
-    static_assert(!BTAB || (LOADCHAIN && !XTAB), "ratio tables serve the load chain");
     const FGeom g{pa.logT, pa.logC, pa.logG};
     uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
     const uint32_t tend = min(t + pa.tpb, pa.ntiles);
     const uint32_t colTiles = g.logT - g.logC;
-    uint32_t tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;   /* the tile worked on */
+    uint32_t tt = t;
     XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0};
-    /* without the load chain the tile goes straight into LDS, in flight
-     * while the stage twiddles are written */
     stage_twiddles<PI, LOGS, 2>(pa.small_tw, nullptr, twi);
     xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
-    if constexpr (!LOADCHAIN) {
+    tile_dma<LOGS>(pa, x, g, tt, lds);
+    if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, true>(xc, g.logC);
+    WAIT_VMCNT(0);
+    lds_sync();
+    for (;;) {
+        dit_all<PI, LOGS, LOGS - 1, true, false>(lds, twi, g.logC, xc);
+        uint32_t tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        size_t base, step;
+        tile_span<LOGS>(pa, g, tt, tid, base, step);
+#pragma unroll
+        for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
+        if (++t >= tend) break;
+        tt = t;
+        lds_sync();
+        xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
         tile_dma<LOGS>(pa, x, g, tt, lds);
         if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, true>(xc, g.logC);
         WAIT_VMCNT(0);
-    }
-    lds_sync();
-    for (;;) {
-        uint32_t tid = threadIdx.x;
-        if constexpr (LOADCHAIN) {
-            uint64_t v[FT_PER];
-            load_tile<LOGS, false>(pa, x, 0, g, tt, v);
-            asm volatile("" : "+v"(tid));     /* recompute index math per tile, not hoisted */
-            const uint32_t r = xc.col0 + (tid & ((1u << g.logC) - 1)), m0 = tid >> g.logC;
-            if constexpr (XTAB) {
-                /* rows m0 + k * 2^(LOGS-3) of column r: one table entry each */
-                const uint64_t *xt = pa.xtab + ((size_t)PI << (LOGS + g.logT)) + ((size_t)m0 << g.logT) + r;
-                uint64_t w[FT_PER];
-#pragma unroll
-                for (int k = 0; k < FT_PER; k++) w[k] = xt[(size_t)k << (LOGS - 3 + g.logT)];
-#pragma unroll
-                for (int k = 0; k < FT_PER; k++) v[k] = mont_mul_lazy<PI>(v[k], w[k]);
-            } else {
-                constexpr uint32_t MASK = (1u << NTT_ROOT_LOG) - 1;
-                uint64_t ta = tw_mont<PI>(xc.tw, (0u - ((r * brev_n(m0, LOGS)) << xc.shift)) & MASK);
-                if (pa.iscale) ta = mont_mul_lazy<PI>(ta, pa.scale_m[PI]);   /* the 1/L rides on the chain */
-                if constexpr (BTAB) {
-                    /* B = w_N^-r and its Shoup partner from the ratio table */
-                    const TwPair b = pa.btab[((size_t)PI << g.logT) + r];
-                    chain_apply_tab<PI, 3>(v, ta, b.w, b.wp);
-                } else {
-                    chain_apply<PI, 3>(v, ta, tw_mont<PI>(xc.tw, (0u - (r << xc.shift)) & MASK));
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < FT_PER; k++) lds[LP(tid + k * FTPB)] = v[k];
-            lds_sync();
-        }
-        /* otherwise the inverse cross twiddles go into the first round; the
-         * table's Montgomery products (an entry below p) leave the tile
-         * below 2p */
-        if constexpr (LOADCHAIN) {
-            /* the top round stores from registers (the inverse passes of a
-             * 2^30-bit product: 1.89 -> 1.83 ms and 2.14 -> 2.11 ms) */
-            dit_all<PI, LOGS, LOGS - 4, false, XTAB>(lds, twi, g.logC, xc);
-            tid = threadIdx.x;
-            asm volatile("" : "+v"(tid));
-            size_t base, step;
-            tile_span<LOGS>(pa, g, tt, tid, base, step);
-            dit_top_store<PI, LOGS, dit_bound(LOGS - 4, XTAB ? 2 : 4)>(lds, twi, g.logC, x, base, step);
-        } else {
-            dit_all<PI, LOGS, LOGS - 1, !LOADCHAIN, XTAB>(lds, twi, g.logC, xc);
-            tid = threadIdx.x;
-            asm volatile("" : "+v"(tid));
-            size_t base, step;
-            tile_span<LOGS>(pa, g, tt, tid, base, step);
-#pragma unroll
-            for (int k = 0; k < FT_PER; k++) x[base + k * step] = lds[LP(tid + k * FTPB)];
-        }
-        if (++t >= tend) break;
-        tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;
         lds_sync();
-        xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
-        if constexpr (!LOADCHAIN) {
-            tile_dma<LOGS>(pa, x, g, tt, lds);
-            if constexpr (xt_ahead<LOGS>()) xt_lookup<PI, LOGS, true>(xc, g.logC);
-            WAIT_VMCNT(0);
-            lds_sync();
-        }
     }
 }
 
@@ -1151,11 +1155,11 @@ __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, cons
     if constexpr (KIND == 0) fwd_tiles<PI, LOGS, false, false>(pa, xa, op, lds, tw);
     else if constexpr (KIND == 6) fwd_tiles<PI, LOGS, false, true>(pa, xa, op, lds, tw);
     else if constexpr (KIND == 3) fwd_tiles<PI, LOGS, true, false>(pa, xa, op, lds, tw);
-    else if constexpr (KIND == 1) inv_tiles<PI, LOGS, false, false>(pa, xa, lds, tw);
-    else if constexpr (KIND == 5) inv_tiles<PI, LOGS, true, false>(pa, xa, lds, tw);
-    else if constexpr (KIND == 7) inv_tiles<PI, LOGS, true, true>(pa, xa, lds, tw);
+    else if constexpr (KIND == 1) inv_tiles<PI, LOGS>(pa, xa, lds, tw);
+    else if constexpr (KIND == 5) inv_tiles_dma<PI, LOGS, false, false>(pa, xa, lds, tw);
+    else if constexpr (KIND == 7) inv_tiles_dma<PI, LOGS, true, false>(pa, xa, lds, tw);
     else if constexpr (KIND == 8) fwd_tiles<PI, LOGS, true, false, true>(pa, xa, op, lds, tw);
-    else if constexpr (KIND == 9) inv_tiles<PI, LOGS, true, false, true>(pa, xa, lds, tw);
+    else if constexpr (KIND == 9) inv_tiles_dma<PI, LOGS, false, true>(pa, xa, lds, tw);
     else if constexpr (KIND == 2) mid_tile<PI, LOGS, false>(pa, xa, xb, lds, tw);
     else mid_tile<PI, LOGS, true>(pa, xa, xb, lds, tw);
 }
