@@ -145,10 +145,13 @@ def test_three_pass_transform(na, nb):
 @pytest.mark.parametrize("na,nb", [(1 << 18, (1 << 18) + 5), (3 << 17, 1 << 16), (5000, 3000), (1 << 22, 1 << 22)])
 def test_cross_twiddle_tables_match_chains(na, nb):
     """Cross twiddles from per-block tables (forward column passes at store,
-    inverse passes at load) against the in-pass Montgomery chains: the same
-    products, both equal to GMP.  Lengths 2^21 (three passes of 2^7: the
-    middle block of 2^14 has tables), 2^15 (two passes: the inverse of the
-    first pass, a 2^15 block, has them) and 2^25 (three passes, 2^17 block)."""
+    inverse passes at load) and chains with their ratios from per-column
+    tables (the first pass, the last inverse pass of blocks above 2^20
+    residues) against the in-pass Montgomery chains with every ratio computed:
+    the same products, both equal to GMP.  Lengths 2^21 (three passes: the
+    middle block has tables), 2^15 (two passes: the inverse of the first pass,
+    a 2^15 block, has them) and 2^25 (three passes; its inverse table pass
+    takes 6 tiles per block over 8192 tiles, the last blocks fewer)."""
     E.init()
     a = O.splitmix(301 + na, na)
     b = O.splitmix(303 + nb, nb)
