@@ -419,14 +419,17 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
      * column slice, and the second one halves the stage-twiddle staging
      * (measured 3.75 -> 3.62 ms at L = 2^27; the other kinds are slower with
      * two) */
-    /* The load-time inverse passes (kinds 5, 7, 9) start each next tile's
-     * transfer during the current tile's top round, and take tiles nblk
-     * apart, so neighbouring column tiles still run side by side: measured at
-     * L = 2^27 (ms, 1 / 2 / 4 / 8 tiles per block) kind 7 1.75 / 1.71 / 1.71 /
-     * 1.67, kind 9 1.92 / 1.87 / 1.87 / 1.88 (1.78 and 1.94 with the
+    /* The wave-local fused pass (kind 2 at S = 2^6, 2^9) and the load-time
+     * inverse passes (kinds 5, 7, 9) start each next tile's transfer during
+     * the current tile's top round, and take tiles nblk apart, so
+     * neighbouring column tiles still run side by side: measured at L = 2^27
+     * (ms, 1 / 2 / 4 / 8 tiles per block) kind 2 3.11 / 3.08 / 3.04 / 3.02-3.05
+     * (3.10-3.11 one tile per block before), kind 7 1.75 / 1.71 / 1.71 / 1.67,
+     * kind 9 1.92 / 1.87 / 1.87 / 1.88 (1.78 and 1.94 with the
      * register-loaded tiles before). */
     auto tiles_per_block = [&](size_t nt, int kind) {
-        const size_t cap = kind == 6   ? std::max(g.tile_batch, 2)
+        const size_t cap = kind == 2   ? std::max(g.tile_batch, 8)
+                           : kind == 6 ? std::max(g.tile_batch, 2)
                            : kind == 7 ? std::max(g.tile_batch, 8)
                            : (kind == 5 || kind == 9) ? std::max(g.tile_batch, 4)
                                                       : g.tile_batch;
@@ -501,6 +504,8 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     pa.bres = b_res;
     pa.blen = b_res ? spectrum_rows_len(logL) : 0;
     pa.scale_late = scale_late;
+    /* the wave-local fused pass (S = 2^6, 2^9) takes several tiles per block */
+    pa.tpb = (!pa.first && (pp.logS[pp.npass - 1] == 6 || pp.logS[pp.npass - 1] == 9)) ? tiles_per_block(ntiles, 2) : 1;
     HIP_TRY(launch_fast_pass(pp.logS[pp.npass - 1], pa.first ? 4 : 2, pa, ntiles, nops, st));
     pa.b_tiles = 0;
     pa.bres = nullptr;

@@ -1138,6 +1138,100 @@ __device__ void mid_tile(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, u
     }
 }
 
+/* The wave-local fused pass over several tiles (kind 2, S = 2^6 or 2^9; the
+ * rounds, transfers and stores of mid_tile's wave-local form).  A block takes
+ * tiles j, j + nblk, ... (j its XCD-ordered index) and stages the stage
+ * twiddles once.  A wave issues its part of the next tile's transfer of a as
+ * soon as it has read its groups of the top inverse round (its own 512 tile
+ * words: no block barrier), so the transfer is in flight during that round's
+ * butterflies and stores. */
+template <int PI, int LOGS>
+__device__ void mid_tiles_wl(const PassArgs &pa, uint64_t *xa, const uint64_t *xb, uint64_t *lds, TwPair *tw)
+{
+    constexpr int RLL = 3, RR = 8, LNB = LOGS - RLL;
+    static_assert(LOGS % 3 == 0 && LOGS >= 6 && (FT_SIZE >> RLL) == FTPB, "8-point groups, one per thread");
+    static_assert((FT_SIZE / 128 / (FTPB / 64)) * 128 == 512, "a wave's tile chunks are its 512 words");
+    const FGeom g{0, 0, pa.logG};
+    const uint32_t nblk = gridDim.x;
+    uint32_t t = xcd_tile(blockIdx.x, nblk);
+    /* a's first tile is in flight while the stage twiddles are written */
+    tile_dma<LOGS>(pa, xa, g, t, lds);
+    stage_twiddles_mid<PI, LOGS, RLL>(pa.small_tw, tw);
+    WAIT_VMCNT(0);
+    lds_sync();      /* the stage twiddles and a's tile */
+    const uint64_t s = pa.scale[PI], sp = pa.scale_p[PI];
+    for (;;) {
+        dif_upper<PI, LOGS, LOGS - 1, RLL, true>(lds, tw);
+        /* a's smallest round, kept in registers: its group leaves LDS first */
+        uint32_t gid = threadIdx.x;
+        asm volatile("" : "+v"(gid));
+        const uint32_t gbase = ((gid >> LNB) << LOGS) + ((gid & ((1u << LNB) - 1)) << RLL);
+        uint64_t ra[RR];
+#pragma unroll
+        for (int i = 0; i < RR; i++) ra[i] = lds[LP(gbase + i)];
+        if (!pa.square) {
+            wave_lds_sync();   /* a's reads are done before b's tile overwrites them */
+            tile_dma<LOGS>(pa, xb, g, pa.b_tiles ? t % pa.b_tiles : t, lds);
+        }
+        /* a's round between the transfer's issue and its wait (pinned there) */
+#pragma unroll
+        for (int i = 0; i < RR; i++) asm volatile("" : "+v"(ra[i]));
+        const_dif<PI, RLL, LZ_MAX>(ra);
+#pragma unroll
+        for (int i = 0; i < RR; i++) asm volatile("" : "+v"(ra[i]));
+        uint64_t x[RR];
+        if (pa.square) {      /* the second factor below 2p */
+#pragma unroll
+            for (int i = 0; i < RR; i++) x[i] = reduce_any<PI>(ra[i]);
+        } else {
+            WAIT_VMCNT(0);
+            wave_lds_sync();
+            dif_upper<PI, LOGS, LOGS - 1, RLL, true>(lds, tw);
+#pragma unroll
+            for (int i = 0; i < RR; i++) x[i] = lds[LP(gbase + i)];
+            const_dif<PI, RLL, 2>(x);
+        }
+        if (pa.scale_late) {    /* the last inverse pass applies the 1/L scale */
+#pragma unroll
+            for (int i = 0; i < RR; i++) x[i] = mont_mul_lazy<PI>(ra[i], x[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < RR; i++) x[i] = shoup_lazy<PI>(mont_mul_lazy<PI>(ra[i], x[i]), s, sp);
+        }
+        const_dit<PI, RLL, 4, LZ_MAX>(x);     /* the products are below 3p */
+#pragma unroll
+        for (int i = 0; i < RR; i++) lds[LP(gbase + i)] = x[i];
+        wave_lds_sync();
+        /* the inverse rounds above the smallest but for the top one */
+        dit_upper<PI, LOGS, LOGS - 4, RLL, true, false>(lds, tw);
+        /* the top round (half-size 2^(LOGS-1), groups of rows j + 2^(LOGS-3) i
+         * of sub-transform gg), stored from registers (T = 1: the tile is
+         * contiguous in its row) */
+        constexpr int LH = LOGS - 1, LD = LH - 2;
+        gid = threadIdx.x;
+        asm volatile("" : "+v"(gid));
+        const uint32_t j = gid & ((1u << LD) - 1);
+        const uint32_t tbase = ((gid >> LD) << LOGS) + j;
+        uint64_t v[RR];
+#pragma unroll
+        for (int i = 0; i < RR; i++) v[i] = lds[LP(tbase + ((uint32_t)i << LD))];
+        lz_group<PI, 3, true, false, dit_bound(LH - 3, 4), LZ_MAX>(v, TabTwF<LH, LD, RR, true, RLL>{tw, j});
+        uint32_t m, r;
+        uint64_t *gout = xa + res_at(ft_addr(LOGS, g, t, 0, m, r), pa.pad_log);
+        t += nblk;
+        const bool more = t < pa.ntiles;
+        if (more) {
+            wave_lds_sync();   /* this wave's words read: its part of the buffer is free */
+            tile_dma<LOGS>(pa, xa, g, t, lds);
+        }
+#pragma unroll
+        for (int i = 0; i < RR; i++) gout[tbase + ((uint32_t)i << LD)] = v[i];
+        if (!more) return;
+        WAIT_VMCNT(0);
+        wave_lds_sync();
+    }
+}
+
 #define TW_LDS_MAX 1024  /* stage-twiddle entries per direction held in LDS (S - 1) */
 
 /* kind 0: forward pass, 1: inverse pass, 2: fused middle pass,
@@ -1160,6 +1254,7 @@ __device__ __forceinline__ void pass_body(const PassArgs &pa, uint64_t *xa, cons
     else if constexpr (KIND == 7) inv_tiles_dma<PI, LOGS, true, false>(pa, xa, lds, tw);
     else if constexpr (KIND == 8) fwd_tiles<PI, LOGS, true, false, true>(pa, xa, op, lds, tw);
     else if constexpr (KIND == 9) inv_tiles_dma<PI, LOGS, false, true>(pa, xa, lds, tw);
+    else if constexpr (KIND == 2 && LOGS % 3 == 0 && LOGS >= 6) mid_tiles_wl<PI, LOGS>(pa, xa, xb, lds, tw);
     else if constexpr (KIND == 2) mid_tile<PI, LOGS, false>(pa, xa, xb, lds, tw);
     else mid_tile<PI, LOGS, true>(pa, xa, xb, lds, tw);
 }
@@ -1174,16 +1269,15 @@ constexpr uint32_t tw_lds_entries()
                                     : (1u << LOGS);
 }
 
-/* The fused middle pass of 512 points (kind 2, S = 2^9: the passes of the
- * longest products) asks for 8 waves per SIMD: at 40 KB of LDS four blocks
- * fit a CU, and the register allocation keeps it at 64 VGPRs without
- * spilling.  So does the 512-point inverse pass with a ratio table (kind 9:
- * 60 VGPRs, 66 by default, where kind 5 takes 60).  Everything else keeps
- * its allocation (capped at 64 VGPRs the other kinds, and the fused pass at
- * other sizes, spill). */
+/* The 512-point inverse pass with a ratio table (kind 9) asks for 8 waves per
+ * SIMD: at 40 KB of LDS four blocks fit a CU, and the allocation keeps it at
+ * 60 VGPRs without spilling (66 by default; kind 5 takes 60).  Everything
+ * else keeps its allocation: capped at 64 VGPRs the other kinds spill, and so
+ * does the multi-tile fused pass (18 registers), which runs faster at 70
+ * VGPRs and three blocks per CU than the single-tile one did at 64 and four
+ * (3.02-3.05 against 3.10-3.11 ms at L = 2^27). */
 template <int LOGS, int KIND>
-__global__ void __launch_bounds__(FTPB)
-    __attribute__((amdgpu_waves_per_eu((KIND == 2 || KIND == 9) && LOGS == 9 ? 8 : 1)))
+__global__ void __launch_bounds__(FTPB) __attribute__((amdgpu_waves_per_eu(KIND == 9 && LOGS == 9 ? 8 : 1)))
 k_fast_pass(PassArgs pa)
 {
     __shared__ uint64_t lds[FT_LDS];
@@ -1204,7 +1298,7 @@ hipError_t launch_fast_pass(int logS, int kind, const PassArgs &pa, size_t ntile
 {
     /* forward and inverse passes: tpb tiles per block; fused passes: one */
     const bool looped = kind == 0 || kind == 1 || kind == 3 || kind == 5 || kind == 6 || kind == 7 || kind == 8 ||
-                        kind == 9;
+                        kind == 9 || (kind == 2 && (logS == 6 || logS == 9));
     if (looped && (pa.tpb < 1 || pa.ntiles != ntiles)) return hipErrorInvalidValue;
     if ((kind == 5 || kind == 7 || kind == 9) && (logS < 3 || pa.logC + logS != 12)) return hipErrorInvalidValue;
     /* ratio tables: the first pass with 8-point last-round groups, and the
