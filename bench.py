@@ -117,7 +117,7 @@ def launch_table(L, logL, na, nb, square=False):
 
     def blocks(kind):
         """Blocks per (operand, prime) row: tiles per block as enqueue_ntt_class sets them."""
-        cap = max(batch, 2) if kind == 6 else batch
+        cap = {6: max(batch, 2), 2: max(batch, 8), 7: max(batch, 8), 5: max(batch, 4), 9: max(batch, 4)}.get(kind, batch)
         tpb = min(max(ntiles * K_PRIMES // 4096, 1), cap)
         return -(-ntiles // tpb)
 
@@ -128,7 +128,9 @@ def launch_table(L, logL, na, nb, square=False):
         byt = 8 * (na + (0 if square else nb)) + nops * K_PRIMES * 8 * Lw if i == 0 else nops * K_PRIMES * 16 * Lw
         out.append((f"k_fast_pass<{s}, {kind}>", blocks(kind) * K_PRIMES * nops, byt))
         done += s
-    out.append((f"k_fast_pass<{plan[-1]}, {2 if len(plan) > 1 else 4}>", ntiles * K_PRIMES,
+    # the wave-local fused pass (S = 2^6, 2^9) takes several tiles per block
+    mid_blocks = blocks(2) if len(plan) > 1 and plan[-1] in (6, 9) else ntiles
+    out.append((f"k_fast_pass<{plan[-1]}, {2 if len(plan) > 1 else 4}>", mid_blocks * K_PRIMES,
                 K_PRIMES * (16 if square else 24) * Lw))
     for i in range(len(plan) - 2, -1, -1):
         done -= plan[i]
