@@ -134,8 +134,11 @@ int ntt_engine_set_tile_batch(int tiles);
 int ntt_engine_get_tile_batch(void);
 /* Cross twiddles of passes whose column block has at most 2^20 residues:
  * from precomputed per-block tables (1, the default; one Montgomery product
- * per residue) or from Montgomery chains inside the pass (0).  Both give the
- * same products.  Returns the previous setting. */
+ * per residue) or from Montgomery chains inside the pass (0).  With 1, the
+ * chains of the other passes (the first forward pass, the inverse passes of
+ * larger blocks) take their ratio from a per-column table and step by Shoup
+ * products; with 0 they compute it.  Both give the same products.  Returns
+ * the previous setting. */
 int ntt_engine_set_xtab(int on);
 
 /* Autotuner: times the schoolbook/NTT crossover on batches of equal-size
