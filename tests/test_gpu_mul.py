@@ -9,6 +9,7 @@ import pytest
 
 import _oracle as O
 import cryptography_amd as E
+from cryptography_amd import engine as EN
 
 pytestmark = pytest.mark.gpu
 
@@ -142,20 +143,29 @@ def test_three_pass_transform(na, nb):
     assert np.array_equal(E.sqr(a), O.gmp_mul(a, a))
 
 
-@pytest.mark.parametrize("na,nb", [(1 << 18, (1 << 18) + 5), (3 << 17, 1 << 16), (5000, 3000), (1 << 22, 1 << 22)])
-def test_cross_twiddle_tables_match_chains(na, nb):
+@pytest.mark.parametrize("na,nb,split", [(1 << 18, (1 << 18) + 5, None), (3 << 17, 1 << 16, None), (5000, 3000, None),
+                                         (1 << 22, 1 << 22, None), (1 << 17, 1 << 17, (6, 8, 6)),
+                                         (1 << 19, (1 << 19) - 7, (8, 8, 6))])
+def test_cross_twiddle_tables_match_chains(na, nb, split):
     """Cross twiddles from per-block tables (forward column passes at store,
     inverse passes at load) and chains with their ratios from per-column
     tables (the first pass, the last inverse pass of blocks above 2^20
     residues) against the in-pass Montgomery chains with every ratio computed:
     the same products, both equal to GMP.  Lengths 2^21 (three passes: the
     middle block has tables), 2^15 (two passes: the inverse of the first pass,
-    a 2^15 block, has them) and 2^25 (three passes; its inverse table pass
-    takes 6 tiles per block over 8192 tiles, the last blocks fewer)."""
+    a 2^15 block, has them), 2^25 (three passes; its inverse table pass
+    takes 6 tiles per block over 8192 tiles, the last blocks fewer), 2^20 split
+    6/8/6 (a 64-point first pass with a ratio table) and 2^22 split 8/8/6 (the
+    last inverse pass, a 2^22 block, by chain with a ratio table at 256
+    points)."""
     E.init()
     a = O.splitmix(301 + na, na)
     b = O.splitmix(303 + nb, nb)
     want_ab, want_aa = O.gmp_mul(a, b), O.gmp_mul(a, a)
+    log_len = E.log_len(na, nb)
+    if split:
+        assert sum(split) == log_len
+        EN.set_pass_split(log_len, split)
     old = E.engine.set_xtab(False)
     try:
         chain_ab, chain_aa = E.mul(a, b), E.sqr(a)
@@ -163,6 +173,8 @@ def test_cross_twiddle_tables_match_chains(na, nb):
         tab_ab, tab_aa = E.mul(a, b), E.sqr(a)
     finally:
         E.engine.set_xtab(old)
+        if split:
+            EN.set_pass_split(log_len, ())
     assert np.array_equal(chain_ab, want_ab) and np.array_equal(chain_aa, want_aa)
     assert np.array_equal(tab_ab, want_ab) and np.array_equal(tab_aa, want_aa)
 
