@@ -490,8 +490,9 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     }
     if (fwd_only) return NTT_OK;         /* the rows hold a's partial spectrum */
     /* the last inverse pass (pass 0's) applies the 1/L scale when it loads
-     * its tile with the cross twiddles (kinds 5 and 7: its chain's first
-     * twiddle or its table carries the scale); otherwise the middle pass does */
+     * its tile with the cross twiddles (kinds 5 and 9: its chain's first
+     * twiddle carries the scale; kind 7: its table does); otherwise the
+     * middle pass does */
     geom(0, 0);
     const bool scale_late = pp.npass >= 2 && pp.logS[0] >= 3 && pa.logC + pp.logS[0] == 12;
     geom(pp.npass - 1, done);

@@ -1269,15 +1269,14 @@ constexpr uint32_t tw_lds_entries()
                                     : (1u << LOGS);
 }
 
-/* The 512-point inverse pass with a ratio table (kind 9) asks for 8 waves per
- * SIMD: at 40 KB of LDS four blocks fit a CU, and the allocation keeps it at
- * 60 VGPRs without spilling (66 by default; kind 5 takes 60).  Everything
- * else keeps its allocation: capped at 64 VGPRs the other kinds spill, and so
- * does the multi-tile fused pass (18 registers), which runs faster at 70
- * VGPRs and three blocks per CU than the single-tile one did at 64 and four
- * (3.02-3.05 against 3.10-3.11 ms at L = 2^27). */
+/* Every kind keeps the compiler's register allocation.  The 512-point passes
+ * at 64 VGPRs or fewer (the inverse passes, 62) run four blocks per CU (40 KB
+ * of LDS each); capped at 64 the others spill, among them the multi-tile
+ * fused pass (18 registers), which runs faster at 70 VGPRs and three blocks
+ * per CU than the single-tile one did at 64 and four (3.02-3.05 against
+ * 3.10-3.11 ms at L = 2^27). */
 template <int LOGS, int KIND>
-__global__ void __launch_bounds__(FTPB) __attribute__((amdgpu_waves_per_eu(KIND == 9 && LOGS == 9 ? 8 : 1)))
+__global__ void __launch_bounds__(FTPB)
 k_fast_pass(PassArgs pa)
 {
     __shared__ uint64_t lds[FT_LDS];

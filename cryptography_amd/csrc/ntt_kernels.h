@@ -64,8 +64,8 @@ struct PassArgs {
     int scale_late;             /* middle pass: leave the 1/L scale to the last inverse pass */
     int half_zero;              /* first pass: every operand has at most L/2 coefficients, so
                                    rows S/2.. of each column are zero */
-    int iscale;                 /* kind 5: multiply by 1/L (the chain's first twiddle carries it);
-                                   kind 7 reads a table that includes it */
+    int iscale;                 /* kinds 5 and 9: multiply by 1/L (the chain's first twiddle
+                                   carries it); kind 7 reads a table that includes it */
     const uint64_t *xtab;       /* kinds 6 and 7: this pass's cross twiddles, one
                                    entry per residue of an S x T block, per prime */
     const TwPair *btab;         /* kinds 8 and 9: the cross-twiddle chain's ratio B
