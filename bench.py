@@ -117,7 +117,7 @@ def launch_table(L, logL, na, nb, square=False):
 
     def blocks(kind):
         """Blocks per (operand, prime) row: tiles per block as enqueue_ntt_class sets them."""
-        cap = {6: max(batch, 2), 2: max(batch, 8), 7: max(batch, 8), 5: max(batch, 4), 9: max(batch, 4)}.get(kind, batch)
+        cap = {6: max(batch, 8), 2: max(batch, 8), 7: max(batch, 8), 5: max(batch, 4), 9: max(batch, 4)}.get(kind, batch)
         tpb = min(max(ntiles * K_PRIMES // 4096, 1), cap)
         return -(-ntiles // tpb)
 

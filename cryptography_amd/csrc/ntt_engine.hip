@@ -413,23 +413,20 @@ int enqueue_ntt_class(Workspace &ws, const MulDesc *d_desc, const MulDesc &one, 
     pa.logL = logL;
     pa.square = square;
     pa.ntiles = (uint32_t)ntiles;
-    /* several tiles per block once the grid has blocks to spare (about 4096
-     * blocks per prime row fill the chip several times over); the forward
-     * table pass (kind 6) takes at least two: its consecutive tiles share a
-     * column slice, and the second one halves the stage-twiddle staging
-     * (measured 3.75 -> 3.62 ms at L = 2^27; the other kinds are slower with
-     * two) */
-    /* The wave-local fused pass (kind 2 at S = 2^6, 2^9) and the load-time
-     * inverse passes (kinds 5, 7, 9) start each next tile's transfer during
-     * the current tile's top round, and take tiles nblk apart, so
-     * neighbouring column tiles still run side by side: measured at L = 2^27
-     * (ms, 1 / 2 / 4 / 8 tiles per block) kind 2 3.11 / 3.08 / 3.04 / 3.02-3.05
-     * (3.10-3.11 one tile per block before), kind 7 1.75 / 1.71 / 1.71 / 1.67,
-     * kind 9 1.92 / 1.87 / 1.87 / 1.88 (1.78 and 1.94 with the
-     * register-loaded tiles before). */
+    /* Several tiles per block once the grid has blocks to spare (about 4096
+     * blocks per prime row fill the chip several times over).  The forward
+     * table pass (kind 6), the wave-local fused pass (kind 2 at S = 2^6, 2^9)
+     * and the load-time inverse passes (kinds 5, 7, 9) start each next tile's
+     * transfer during the current tile's last round, and take tiles nblk
+     * apart, so neighbouring column tiles still run side by side: measured at
+     * L = 2^27 (ms, 1 / 2 / 4 / 8 tiles per block) kind 6 with consecutive
+     * tiles 3.75 / 3.62 (two), strided 3.24-3.25 at two and 3.20-3.22 at
+     * eight, kind 2 3.11 / 3.08 / 3.04 / 3.02-3.05, kind 7 1.75 / 1.71 /
+     * 1.71 / 1.67, kind 9 1.92 / 1.87 / 1.87 / 1.88.  The first pass is
+     * slower with more than one (3.18 against 3.21-3.41 ms). */
     auto tiles_per_block = [&](size_t nt, int kind) {
         const size_t cap = kind == 2   ? std::max(g.tile_batch, 8)
-                           : kind == 6 ? std::max(g.tile_batch, 2)
+                           : kind == 6 ? std::max(g.tile_batch, 8)
                            : kind == 7 ? std::max(g.tile_batch, 8)
                            : (kind == 5 || kind == 9) ? std::max(g.tile_batch, 4)
                                                       : g.tile_batch;

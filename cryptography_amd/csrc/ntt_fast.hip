@@ -720,8 +720,11 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
 {
     static_assert(!BTAB || (FIRST && LOGS >= 4 && xt_ahead<LOGS>()), "ratio tables: first pass, B ahead");
     const FGeom g{pa.logT, pa.logC, pa.logG};
-    uint32_t t = xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
-    const uint32_t tend = min(t + pa.tpb, pa.ntiles);
+    /* the table pass takes tiles nblk apart (as inv_tiles_dma: neighbouring
+     * column tiles run side by side), the others consecutive ones */
+    constexpr bool STRIDED = XTAB && !FIRST && LOGS >= 3;   /* the loop that takes them */
+    uint32_t t = STRIDED ? xcd_tile(blockIdx.x, gridDim.x) : xcd_tile(blockIdx.x, gridDim.x) * pa.tpb;
+    const uint32_t tend = STRIDED ? pa.ntiles : min(t + pa.tpb, pa.ntiles);
     const uint32_t colTiles = g.logT - g.logC;
     uint32_t tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;   /* the tile worked on */
     XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0, 0,
@@ -765,9 +768,11 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
                 for (int i = 0; i < R; i++) w[n * R + i] = xt[((size_t)(m0 + i) << g.logT) + xc.col0 + c];
             }
             const uint32_t tcur = tt;
-            const bool more = t + 1 < tend;
+            const uint32_t tnext = t + (STRIDED ? gridDim.x : 1);
+            const bool more = tnext < tend;
             if (more) {
-                tt = xtab_tile(++t, colTiles, pa.ntiles);
+                t = tnext;
+                tt = xtab_tile(t, colTiles, pa.ntiles);
                 lds_sync();       /* every group read: the tile buffer is free */
                 tile_dma<LOGS>(pa, x, g, tt, lds);
             }

@@ -129,7 +129,7 @@ int ntt_engine_set_pass_split(int log_len, const int *log_s, int npass);
 int ntt_engine_get_plan(int log_len, int *log_s, int max);
 /* Most tiles (4096 residues each) one block of a pass transforms in turn
  * (default 1, range 1..64; applied to large transforms only, small ones use
- * 1).  Some passes take more whatever the setting: the forward table pass 2,
+ * 1).  Some passes take more whatever the setting: the forward table pass,
  * the 64- and 512-point fused pass and the inverse table pass 8, the other
  * inverse passes that apply their cross twiddles at the load 4. */
 int ntt_engine_set_tile_batch(int tiles);
