@@ -158,6 +158,21 @@ __device__ __forceinline__ void tile_span(const PassArgs &pa, const FGeom &g, ui
     step = res_at(ft_addr(LOGS, g, t, tid + FTPB, m, r), pa.pad_log) - base;
 }
 
+/* Group store x[gb + i*gs] = v[i], i < R, by a running pointer: one 64-bit
+ * add per store.  gs is a per-lane value to the compiler, which otherwise
+ * forms each i*gs with two 64-bit multiplies and two moves. */
+template <int R>
+__device__ __forceinline__ void group_store(uint64_t *x, size_t gb, size_t gs, const uint64_t *v)
+{
+    uint64_t *xp = x + gb;
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        *xp = v[i];
+        xp += gs;
+        asm("" : "+v"(xp));     /* keeps the adds from being folded back into i*gs */
+    }
+}
+
 /* Forward group of a compile-time round, inputs below 2p. */
 template <int PI, int RL, int BOUT>
 __device__ __forceinline__ void const_dif(uint64_t *x)
@@ -534,8 +549,7 @@ __device__ __forceinline__ void dif_last_store(const PassArgs &pa, const FGeom &
             const uint32_t r = xc.col0 + c;
             chain_twiddle<PI, RL>(v, xc.tw, (r * brev_n(m0, LOGS)) << xc.shift, (r << (LOGS - RL)) << xc.shift);
         }
-#pragma unroll
-        for (int i = 0; i < R; i++) x[gb + i * gs] = v[i];
+        group_store<R>(x, gb, gs, v);
     }
 }
 
@@ -550,8 +564,7 @@ __device__ __forceinline__ void dit_top_finish(uint64_t *v, const TwPair *twi, u
     const uint32_t j = threadIdx.x >> logC;    /* the group's first row, below S/8 = 2^LD */
     /* inputs below BIN*p, stored unreduced, below 16p (inverse-pass output) */
     lz_group<PI, RL, true, false, BIN, LZ_MAX>(v, TabTwF<LH, LD, R, true, 0>{twi, j});
-#pragma unroll
-    for (int i = 0; i < R; i++) x[gb + i * gs] = v[i];
+    group_store<R>(x, gb, gs, v);
 }
 
 /* All DIF rounds of an S = 2^LOGS transform, top stage first, radix 8 while
@@ -685,19 +698,29 @@ __device__ __forceinline__ void fill_first_tile(const PassArgs &pa, int op, cons
  * otherwise never occurs; the launcher checks).  The caller waits (vmcnt)
  * and syncs before reading the tile. */
 typedef __attribute__((address_space(3))) void lds_void_t;
-template <int LOGS>
+/* SPLIT: word q0 + 2*lane of chunk q0 (a multiple of 128; 2*lane below 128)
+ * is addressed as ft_addr(t, q0) + ft_addr(0, 2*lane).  ft_addr's fields c,
+ * m and gg are bit ranges of q and the two parts occupy disjoint bits, so
+ * the sum is ft_addr(t, q0 + 2*lane).  The first part is wave-uniform
+ * (scalar registers: t is the block's tile), the second a per-lane offset:
+ * about 4 VALU instructions per transfer instead of 17 (forward table pass
+ * 3.24 -> 3.19 ms).  The chain pass with the ratio table measured slower
+ * with it (1.81 -> 1.88 ms) and takes the per-lane form (!SPLIT). */
+template <int LOGS, bool SPLIT = true>
 __device__ __forceinline__ void tile_dma(const PassArgs &pa, const uint64_t *x, const FGeom &g, uint32_t t,
                                          uint64_t *lds)
 {
     constexpr int CHUNKS = FT_SIZE / 128 / (FTPB / 64);    /* 1 KB chunks per wave */
     uint32_t tid = threadIdx.x;
     asm volatile("" : "+v"(tid));     /* address math per call, not hoisted out of tile loops */
-    const uint32_t lane = tid & 63, w = tid >> 6;
+    const uint32_t w = SPLIT ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid >> 6;
+    uint32_t m, r;
+    const size_t lo = SPLIT ? ft_addr(LOGS, g, 0, 2 * (tid & 63), m, r) : 0;
 #pragma unroll
     for (int j = 0; j < CHUNKS; j++) {
         const uint32_t q0 = (w * CHUNKS + j) * 128;
-        uint32_t m, r;
-        const size_t a = res_at(ft_addr(LOGS, g, t, q0 + 2 * lane, m, r), pa.pad_log);
+        const size_t a = SPLIT ? res_at(ft_addr(LOGS, g, t, q0, m, r) + lo, pa.pad_log)
+                               : res_at(ft_addr(LOGS, g, t, q0 + 2 * (tid & 63), m, r), pa.pad_log);
         __builtin_amdgcn_global_load_lds((const void *)(x + a), (lds_void_t *)(lds + LP(q0)), 16, 0, 0);
     }
 }
@@ -765,7 +788,8 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
 #pragma unroll
                 for (int i = 0; i < R; i++) v[n * R + i] = lds[LP(q0 + ((uint32_t)i << g.logC))];
 #pragma unroll
-                for (int i = 0; i < R; i++) w[n * R + i] = xt[((size_t)(m0 + i) << g.logT) + xc.col0 + c];
+                for (int i = 0; i < R; i++)
+                    w[n * R + i] = xt[((size_t)m0 << g.logT) + xc.col0 + c + i * ((size_t)1 << g.logT)];
             }
             const uint32_t tcur = tt;
             const uint32_t tnext = t + (STRIDED ? gridDim.x : 1);
@@ -787,7 +811,8 @@ __device__ void fwd_tiles(const PassArgs &pa, uint64_t *x, int op, uint64_t *lds
                  * entry below p returns (0, 2p) */
                 const_dif<PI, RL, LZ_MAX>(v + n * R);
 #pragma unroll
-                for (int i = 0; i < R; i++) x[gb + i * gs] = mont_mul_lazy<PI>(v[n * R + i], w[n * R + i]);
+                for (int i = 0; i < R; i++) v[n * R + i] = mont_mul_lazy<PI>(v[n * R + i], w[n * R + i]);
+                group_store<R>(x, gb, gs, v + n * R);
             }
             if (!more) return;
             xc.col0 = (tt & ((1u << colTiles) - 1)) << g.logC;
@@ -866,7 +891,7 @@ __device__ void inv_tiles_dma(const PassArgs &pa, uint64_t *x, uint64_t *lds, Tw
     uint32_t tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;   /* the tile worked on */
     XtwCtx xc{pa.tw + (size_t)PI * 2 * NTT_TW_SIZE, 0, NTT_ROOT_LOG - (LOGS + g.logT), 0, 0};
     /* the first tile is in flight while the stage twiddles are written */
-    tile_dma<LOGS>(pa, x, g, tt, lds);
+    tile_dma<LOGS, !BTAB>(pa, x, g, tt, lds);
     stage_twiddles<PI, LOGS, 2>(pa.small_tw, nullptr, twi);
     for (;;) {
         uint32_t tid = threadIdx.x;
@@ -879,7 +904,7 @@ __device__ void inv_tiles_dma(const PassArgs &pa, uint64_t *x, uint64_t *lds, Tw
             const uint64_t *xt = pa.xtab + ((size_t)PI << (LOGS + g.logT)) + ((size_t)m0 << g.logT) + r;
             uint64_t w[FT_PER];
 #pragma unroll
-            for (int k = 0; k < FT_PER; k++) w[k] = xt[(size_t)k << (LOGS - 3 + g.logT)];
+            for (int k = 0; k < FT_PER; k++) w[k] = xt[k * ((size_t)1 << (LOGS - 3 + g.logT))];
             WAIT_VMCNT(0);
             lds_sync();
 #pragma unroll
@@ -923,7 +948,7 @@ __device__ void inv_tiles_dma(const PassArgs &pa, uint64_t *x, uint64_t *lds, Tw
         if (more) {
             tt = XTAB ? xtab_tile(t, colTiles, pa.ntiles) : t;
             lds_sync();           /* every group read: the tile buffer is free */
-            tile_dma<LOGS>(pa, x, g, tt, lds);
+            tile_dma<LOGS, !BTAB>(pa, x, g, tt, lds);
         }
         dit_top_finish<PI, LOGS, dit_bound(LOGS - 4, XTAB ? 2 : 4)>(v, twi, g.logC, x, base, step);
         if (!more) return;
