@@ -48,13 +48,9 @@
 #include "lazy_group.h"
 #include "modarith.h"
 #include "ntt_kernels.h"
+#include "tile_geom.h"
 
 using namespace nttc;
-
-#define FT_LOG 12                 /* tile: 4096 residues */
-#define FT_SIZE (1u << FT_LOG)
-#define FTPB 512                  /* threads per block */
-#define FT_PER (FT_SIZE / FTPB)   /* residues per thread */
 
 /* Shoup pairs w_4096^e, e < 2048, per prime: the stage twiddles of every
  * sub-transform of up to 4096 points (w_S^x = w_4096^(x*4096/S)). */
@@ -125,23 +121,6 @@ __device__ __forceinline__ uint32_t xtab_tile(uint32_t t, uint32_t logCT, uint32
 __device__ __forceinline__ uint32_t brev_n(uint32_t x, int bits)
 {
     return bits ? (__brev(x) >> (32 - bits)) : 0;
-}
-
-/* Tile geometry: G blocks x S rows x C columns, LDS index ((g*S + m) << logC) + c. */
-struct FGeom {
-    uint32_t logT, logC, logG;
-};
-
-__device__ __forceinline__ size_t ft_addr(int LOGS, const FGeom &g, uint32_t t, uint32_t q, uint32_t &m, uint32_t &r)
-{
-    uint32_t c = q & ((1u << g.logC) - 1);
-    m = (q >> g.logC) & ((1u << LOGS) - 1);
-    uint32_t gg = q >> (g.logC + LOGS);
-    uint32_t colTiles = g.logT - g.logC;
-    size_t b0 = ((size_t)(t >> colTiles)) << g.logG;
-    uint32_t col0 = (t & ((1u << colTiles) - 1)) << g.logC;
-    r = col0 + c;
-    return ((b0 + gg) << (LOGS + g.logT)) + r + ((size_t)m << g.logT);
 }
 
 /* Residue k of thread tid's share of tile t (q = tid + k * FTPB) sits at
