@@ -475,23 +475,14 @@ __device__ __forceinline__ void dif_above_last(uint64_t *lds, const TwPair *twf,
     }
 }
 
-/* The smallest DIF round of an S = 2^LOGS transform (LOGS >= 1) works on
- * groups of 2^LAST_RL<LOGS> consecutive rows, 8 >> LAST_RL of them per thread. */
-template <int LOGS>
-constexpr int last_rl() { return ((LOGS - 1) % 3) + 1; }
-
-/* Thread tid's group n of that round: rows m0 .. m0 + 2^RL - 1 of tile column
- * c, at tile words q0 + i*C; the same rows go to gb + i*gs in the row. */
+/* Thread tid's group n of the smallest DIF round (groups of 2^last_rl(LOGS)
+ * rows, tile_geom.h): rows m0 .. m0 + 2^RL - 1 of tile column c, at tile
+ * words q0 + i*C; the same rows go to gb + i*gs in the row. */
 template <int LOGS>
 __device__ __forceinline__ uint32_t last_group(const PassArgs &pa, const FGeom &g, uint32_t tt, uint32_t tid, int n,
                                                uint32_t &c, uint32_t &m0, size_t &gb, size_t &gs)
 {
-    constexpr int RL = last_rl<LOGS>();
-    const uint32_t gid = tid + (uint32_t)n * FTPB;
-    c = gid & ((1u << g.logC) - 1);
-    const uint32_t u = gid >> g.logC;
-    m0 = (u & ((1u << (LOGS - RL)) - 1)) << RL;
-    const uint32_t q0 = ((((u >> (LOGS - RL)) << LOGS) + m0) << g.logC) + c;
+    const uint32_t q0 = last_group_q0<LOGS>(g.logC, tid + (uint32_t)n * FTPB, c, m0);
     uint32_t m, r;
     gb = res_at(ft_addr(LOGS, g, tt, q0, m, r), pa.pad_log);
     gs = res_at(ft_addr(LOGS, g, tt, q0 + (1u << g.logC), m, r), pa.pad_log) - gb;

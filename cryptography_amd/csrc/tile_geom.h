@@ -37,4 +37,22 @@ __host__ __device__ __forceinline__ size_t ft_addr(int LOGS, const FGeom &g, uin
     return ((b0 + gg) << (LOGS + g.logT)) + r + ((size_t)m << g.logT);
 }
 
+/* The smallest DIF round of an S = 2^LOGS transform (LOGS >= 1) works on
+ * groups of 2^last_rl(LOGS) consecutive rows, 8 >> last_rl of them per thread. */
+__host__ __device__ constexpr int last_rl(int LOGS) { return ((LOGS - 1) % 3) + 1; }
+template <int LOGS>
+__host__ __device__ constexpr int last_rl() { return last_rl(LOGS); }
+
+/* Group gid (thread tid's n-th: gid = tid + n*FTPB) of that round: rows
+ * m0 .. m0 + 2^RL - 1 of tile column c, at tile words q0 + i*C. */
+template <int LOGS>
+__host__ __device__ __forceinline__ uint32_t last_group_q0(uint32_t logC, uint32_t gid, uint32_t &c, uint32_t &m0)
+{
+    constexpr int RL = last_rl(LOGS);
+    c = gid & ((1u << logC) - 1);
+    const uint32_t u = gid >> logC;
+    m0 = (u & ((1u << (LOGS - RL)) - 1)) << RL;
+    return ((((u >> (LOGS - RL)) << LOGS) + m0) << logC) + c;
+}
+
 #endif

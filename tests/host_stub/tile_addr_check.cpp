@@ -4,8 +4,10 @@
  *  - tile_dma: word q0 + 2*lane of a 1 KB chunk (q0 a multiple of 128) lies
  *    at res_at(ft_addr(t, q0) + ft_addr(0, 2*lane)), the same index as
  *    res_at(ft_addr(t, q0 + 2*lane));
- *  - tile_span / last_group: the words a thread stores lie at base + k*step
- *    (for the row pads the engine pairs with each geometry).
+ *  - tile_span: the words a thread stores lie at base + k*step (for the row
+ *    pads the engine pairs with each geometry);
+ *  - last_group: the smallest DIF round's groups cover the tile once, rows
+ *    m0 .. m0 + 2^RL - 1 of a column, stored at gb + i*gs.
  * Every pass geometry the launcher makes (S = 2^3..2^10 points, T = 1..2^27
  * columns, C = min(T, 4096 / S), G = 4096 / (S C)), tiles at both ends and spread between, and the row pads
  * in use.  Prints "bad N" and exits non-zero on any mismatch.
@@ -47,6 +49,48 @@ static void check_tile(int LOGS, const FGeom &g, uint32_t t, uint32_t pad)
     }
 }
 
+/* The smallest DIF round's groups (last_group, dif_last_store and the table
+ * pass's register stores): the groups of all threads cover every tile word
+ * once, and group rows m0 + i lie at gb + i*gs. */
+template <int LOGS>
+static void check_groups(const FGeom &g, uint32_t t, uint32_t pad)
+{
+    constexpr int RL = last_rl<LOGS>(), R = 1 << RL, NG = 8 >> RL;
+    static unsigned char seen[FT_SIZE];
+    for (uint32_t q = 0; q < FT_SIZE; q++) seen[q] = 0;
+    const bool span = pad == NTT_PAD_NONE || pad == g.logT || pad >= LOGS + g.logT;
+    uint32_t m, r;
+    for (uint32_t tid = 0; tid < FTPB; tid++)
+        for (int n = 0; n < NG; n++) {
+            uint32_t c, m0;
+            const uint32_t q0 = last_group_q0<LOGS>(g.logC, tid + (uint32_t)n * FTPB, c, m0);
+            const size_t gb = res_at(ft_addr(LOGS, g, t, q0, m, r), pad);
+            const size_t gs = res_at(ft_addr(LOGS, g, t, q0 + (1u << g.logC), m, r), pad) - gb;
+            for (int i = 0; i < R; i++) {
+                const uint32_t q = q0 + ((uint32_t)i << g.logC);
+                checks++;
+                if ((q >= FT_SIZE || seen[q]++ || (ft_addr(LOGS, g, t, q, m, r), m != m0 + (uint32_t)i)) && bad++ < 8)
+                    printf("groups: LOGS %d logC %u tid %u n %d i %d q %u\n", LOGS, g.logC, tid, n, i, q);
+                if (span && q < FT_SIZE && res_at(ft_addr(LOGS, g, t, q, m, r), pad) != gb + i * gs && bad++ < 8)
+                    printf("group span: LOGS %d logC %u logT %u t %u tid %u i %d\n", LOGS, g.logC, g.logT, t, tid, i);
+            }
+        }
+}
+
+static void check_groups_any(int LOGS, const FGeom &g, uint32_t t, uint32_t pad)
+{
+    switch (LOGS) {
+    case 3: check_groups<3>(g, t, pad); break;
+    case 4: check_groups<4>(g, t, pad); break;
+    case 5: check_groups<5>(g, t, pad); break;
+    case 6: check_groups<6>(g, t, pad); break;
+    case 7: check_groups<7>(g, t, pad); break;
+    case 8: check_groups<8>(g, t, pad); break;
+    case 9: check_groups<9>(g, t, pad); break;
+    case 10: check_groups<10>(g, t, pad); break;
+    }
+}
+
 int main()
 {
     const uint32_t pads[] = {12, 14, 17, 18, 20, 27, NTT_PAD_NONE};
@@ -60,6 +104,7 @@ int main()
                     uint32_t t = s < 12 ? s : (uint32_t)(((uint64_t)s * 2654435761u) % (1u << 20));
                     if (s == 23) t = (1u << 20) - 1;
                     check_tile(LOGS, g, t, pad);
+                    if (s % 6 == 0 || s == 23) check_groups_any(LOGS, g, t, pad);
                 }
         }
     printf("checks %ld\nbad %ld\n", checks, bad);

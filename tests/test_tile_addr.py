@@ -1,8 +1,9 @@
 """Address identities of the LDS-tiled transform passes
 (cryptography_amd/csrc/tile_geom.h), compiled for the host: the split tile
 transfer address (a wave-uniform chunk part plus a lane offset) equals the
-whole one, and a thread's stored words lie at base + k*step, for every pass
-geometry the launcher makes, tiles at both ends and the row pads in use.  The
+whole one, a thread's stored words lie at base + k*step, and the smallest
+round's register groups cover the tile once and store at gb + i*gs, for every
+pass geometry the launcher makes, tiles at both ends and the row pads in use.  The
 kernels' memory accesses are built on these identities; the GPU tests check
 the products they lead to."""
 import os
