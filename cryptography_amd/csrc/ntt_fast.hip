@@ -484,8 +484,9 @@ __device__ __forceinline__ uint32_t last_group(const PassArgs &pa, const FGeom &
 {
     const uint32_t q0 = last_group_q0<LOGS>(g.logC, tid + (uint32_t)n * FTPB, c, m0);
     uint32_t m, r;
-    gb = res_at(ft_addr(LOGS, g, tt, q0, m, r), pa.pad_log);
-    gs = res_at(ft_addr(LOGS, g, tt, q0 + (1u << g.logC), m, r), pa.pad_log) - gb;
+    const size_t a0 = ft_addr(LOGS, g, tt, q0, m, r);
+    gb = res_at(a0, pa.pad_log);
+    gs = res_at(a0 + ((size_t)1 << g.logT), pa.pad_log) - gb;   /* row m0 + 1: 2^logT residues on */
     return q0;
 }
 

@@ -7,7 +7,8 @@
  *  - tile_span: the words a thread stores lie at base + k*step (for the row
  *    pads the engine pairs with each geometry);
  *  - last_group: the smallest DIF round's groups cover the tile once, rows
- *    m0 .. m0 + 2^RL - 1 of a column, stored at gb + i*gs.
+ *    m0 .. m0 + 2^RL - 1 of a column, stored at gb + i*gs, the next row
+ *    2^logT residues on.
  * Every pass geometry the launcher makes (S = 2^3..2^10 points, T = 1..2^27
  * columns, C = min(T, 4096 / S), G = 4096 / (S C)), tiles at both ends and spread between, and the row pads
  * in use.  Prints "bad N" and exits non-zero on any mismatch.
@@ -65,7 +66,12 @@ static void check_groups(const FGeom &g, uint32_t t, uint32_t pad)
             uint32_t c, m0;
             const uint32_t q0 = last_group_q0<LOGS>(g.logC, tid + (uint32_t)n * FTPB, c, m0);
             const size_t gb = res_at(ft_addr(LOGS, g, t, q0, m, r), pad);
-            const size_t gs = res_at(ft_addr(LOGS, g, t, q0 + (1u << g.logC), m, r), pad) - gb;
+            const size_t a0 = ft_addr(LOGS, g, t, q0, m, r);
+            const size_t gs = res_at(a0 + ((size_t)1 << g.logT), pad) - gb;
+            /* the next row of the group is 2^logT residues on (last_group's stride) */
+            checks++;
+            if (ft_addr(LOGS, g, t, q0 + (1u << g.logC), m, r) != a0 + ((size_t)1 << g.logT) && bad++ < 8)
+                printf("group stride: LOGS %d logC %u logT %u tid %u n %d\n", LOGS, g.logC, g.logT, tid, n);
             for (int i = 0; i < R; i++) {
                 const uint32_t q = q0 + ((uint32_t)i << g.logC);
                 checks++;
